@@ -1,0 +1,322 @@
+"""Python host binding of libtissue (include/tissue.h) over ctypes.
+
+The package directory starts with a digit, so import it with
+``importlib.import_module("2dtissue_amd")``. The library is built in-tree by
+``make -C 2dtissue_amd`` (``__graft_entry__.build()``); loading fails loudly when it is missing:
+there is no CPU fallback.
+
+Reference interface mirrored (KAIZN-io/2DTissue, src/simulation/2DTissue.h:34-129):
+``Simulation.start`` / ``update`` / ``is_finished`` / ``get_order_parameter`` wrap the C-ABI the
+same way include/2DTissue.h does for C++.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtissue.so")
+
+OK = 0
+E_INVALID_ARG = 1
+E_LOST_PARTICLE = 2
+E_NONFINITE = 3
+E_TILING_NO_CONVERGE = 4
+E_HIP = 5
+E_IO = 6
+E_MESH = 7
+FLAG_LOST_PARTICLE = 1
+FLAG_NONFINITE = 2
+FLAG_TILING = 4
+
+_STATUS_NAMES = {1: "E_INVALID_ARG", 2: "E_LOST_PARTICLE", 3: "E_NONFINITE", 4: "E_TILING_NO_CONVERGE",
+                 5: "E_HIP", 6: "E_IO", 7: "E_MESH"}
+
+
+class TissueError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{_STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [("sigma", C.c_double), ("v0", C.c_double), ("k", C.c_double), ("mu", C.c_double),
+                ("tau", C.c_double), ("dt", C.c_double), ("max_crossings", C.c_int32), ("reserved", C.c_int32)]
+
+
+def params(sigma=0.05, v0=1.0, k=10.0, mu=1.0, tau=1.0, dt=0.02, max_crossings=0) -> Params:
+    return Params(sigma, v0, k, mu, tau, dt, max_crossings, 0)
+
+
+_lib = None
+
+# (name, restype, argtypes) of every entry point declared in include/tissue.h
+_P = C.c_void_p
+_F = C.POINTER(C.c_float)
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int32)
+SIGNATURES = [
+    ("tissue_error_string", C.c_char_p, []),
+    ("tissue_mesh_ellipsoid", C.c_int32, [C.c_double, C.c_double, C.c_double, C.c_int32, C.POINTER(_P)]),
+    ("tissue_mesh_perturbed_sphere", C.c_int32, [C.c_double, C.c_int32, C.POINTER(_P)]),
+    ("tissue_mesh_load_off", C.c_int32, [C.c_char_p, C.POINTER(_P)]),
+    ("tissue_mesh_build_tables", C.c_int32, [_P, C.c_double, C.c_int32]),
+    ("tissue_mesh_destroy", None, [_P]),
+    ("tissue_mesh_info", C.c_int32, [_P, _I, _I, _I, C.POINTER(C.c_int64), _I, C.POINTER(C.c_int64)]),
+    ("tissue_mesh_export", C.c_int32, [_P, _D, _D, _I, _I, _I, _F, _I, _I]),
+    ("tissue_mesh_save_off", C.c_int32, [_P, C.c_char_p, C.c_char_p]),
+    ("tissue_create", C.c_int32, [_P, C.POINTER(Params), C.c_int32, C.POINTER(_P)]),
+    ("tissue_destroy", None, [_P]),
+    ("tissue_init_random", C.c_int32, [_P, C.c_uint64]),
+    ("tissue_set_state", C.c_int32, [_P, _F, _F, _F, _I]),
+    ("tissue_get_state", C.c_int32, [_P, _F, _F, _F, _I]),
+    ("tissue_get_outputs", C.c_int32, [_P, _F, _F, _F, _I, _I]),
+    ("tissue_step", C.c_int32, [_P, C.c_int32, _F]),
+    ("tissue_step_device", C.c_int32, [_P, C.c_int32, C.c_void_p]),
+    ("tissue_stage_noise", C.c_int32, [_P, _F, C.c_int32]),
+    ("tissue_step_staged", C.c_int32, [_P, C.c_int32, C.c_int32]),
+    ("tissue_get_order", C.c_int32, [_P, _D, _D, C.c_int32, _I]),
+    ("tissue_reset_order", C.c_int32, [_P]),
+    ("tissue_get_flags", C.c_int32, [_P, C.POINTER(C.c_uint32), _I]),
+    ("tissue_neighbor_lists", C.c_int32, [_P, _I, _I, C.c_int32]),
+    ("tissue_set_launch", C.c_int32, [_P, C.c_int32, C.c_int32]),
+    ("tissue_synchronize", C.c_int32, [_P]),
+    ("tissue_last_step_ms", C.c_int32, [_P, _D]),
+    ("tissue_gen_noise", C.c_int32, [C.c_uint64, C.c_double, C.c_int32, C.c_int32, _F]),
+    ("tissue_philox4x32", None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+]
+
+
+def lib() -> C.CDLL:
+    """Load libtissue.so (raises OSError when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} is missing: build it with `make -C 2dtissue_amd`")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(code: int) -> None:
+    if code != OK:
+        raise TissueError(code, lib().tissue_error_string().decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray | None, ct):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+class Mesh:
+    """UV-parameterised closed surface with its step tables (tissue_mesh)."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    @classmethod
+    def ellipsoid(cls, a=2.0, b=1.0, c=1.0, freq=22) -> "Mesh":
+        h = C.c_void_p()
+        _check(lib().tissue_mesh_ellipsoid(a, b, c, freq, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def perturbed_sphere(cls, amp=0.2, freq=100) -> "Mesh":
+        h = C.c_void_p()
+        _check(lib().tissue_mesh_perturbed_sphere(amp, freq, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def load_off(cls, path: str) -> "Mesh":
+        h = C.c_void_p()
+        _check(lib().tissue_mesh_load_off(path.encode(), C.byref(h)))
+        return cls(h)
+
+    def build_tables(self, sigma_max: float, grid: int = 0) -> "Mesh":
+        _check(lib().tissue_mesh_build_tables(self.h, sigma_max, grid))
+        return self
+
+    def info(self) -> dict:
+        nv, nf, nv3d, grid = (C.c_int32() for _ in range(4))
+        nnz, ncf = C.c_int64(), C.c_int64()
+        _check(lib().tissue_mesh_info(self.h, C.byref(nv), C.byref(nf), C.byref(nv3d), C.byref(nnz), C.byref(grid),
+                                      C.byref(ncf)))
+        return dict(nv=nv.value, nf=nf.value, nv3d=nv3d.value, nnz=nnz.value, grid=grid.value,
+                    n_cell_faces=ncf.value)
+
+    def export(self) -> dict:
+        i = self.info()
+        uv = np.zeros((i["nv"], 2)); xyz = np.zeros((i["nv"], 3))
+        faces = np.zeros((i["nf"], 3), np.int32)
+        row_ptr = np.zeros(i["nv"] + 1, np.int32)
+        col = np.zeros(max(i["nnz"], 1), np.int32); dist = np.zeros(max(i["nnz"], 1), np.float32)
+        g = i["grid"]
+        cell_start = np.zeros(g * g + 1 if g else 1, np.int32)
+        cell_faces = np.zeros(max(i["n_cell_faces"], 1), np.int32)
+        _check(lib().tissue_mesh_export(self.h, _ptr(uv, C.c_double), _ptr(xyz, C.c_double), _ptr(faces, C.c_int32),
+                                        _ptr(row_ptr, C.c_int32), _ptr(col, C.c_int32), _ptr(dist, C.c_float),
+                                        _ptr(cell_start, C.c_int32), _ptr(cell_faces, C.c_int32)))
+        out = dict(i)
+        out.update(uv=uv, xyz=xyz, faces=faces, row_ptr=row_ptr, col=col[: i["nnz"]], dist=dist[: i["nnz"]],
+                   cell_start=cell_start, cell_faces=cell_faces[: i["n_cell_faces"]])
+        return out
+
+    def save_off(self, path_3d: str | None, path_uv: str | None) -> None:
+        _check(lib().tissue_mesh_save_off(self.h, path_3d.encode() if path_3d else None,
+                                          path_uv.encode() if path_uv else None))
+
+    def close(self):
+        if self.h:
+            lib().tissue_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class State:
+    x: np.ndarray
+    y: np.ndarray
+    theta: np.ndarray
+    vid: np.ndarray
+
+
+@dataclass
+class Outputs:
+    vx: np.ndarray
+    vy: np.ndarray
+    r3d: np.ndarray
+    face: np.ndarray
+    nbr: np.ndarray
+
+
+class Sim:
+    """One simulation on the current GPU (tissue_sim)."""
+
+    def __init__(self, mesh: Mesh, p: Params, n: int):
+        self.mesh = mesh  # keeps the mesh alive
+        self.n = n
+        self.p = p
+        h = C.c_void_p()
+        _check(lib().tissue_create(mesh.h, C.byref(p), n, C.byref(h)))
+        self.h = h
+
+    def init_random(self, seed: int) -> None:
+        _check(lib().tissue_init_random(self.h, seed))
+
+    def set_state(self, x, y, theta, vid=None) -> None:
+        x = np.ascontiguousarray(x, np.float32); y = np.ascontiguousarray(y, np.float32)
+        t = np.ascontiguousarray(theta, np.float32)
+        v = None if vid is None else np.ascontiguousarray(vid, np.int32)
+        _check(lib().tissue_set_state(self.h, _ptr(x, C.c_float), _ptr(y, C.c_float), _ptr(t, C.c_float),
+                                      _ptr(v, C.c_int32)))
+
+    def get_state(self) -> State:
+        n = self.n
+        s = State(np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros(n, np.int32))
+        _check(lib().tissue_get_state(self.h, _ptr(s.x, C.c_float), _ptr(s.y, C.c_float), _ptr(s.theta, C.c_float),
+                                      _ptr(s.vid, C.c_int32)))
+        return s
+
+    def get_outputs(self) -> Outputs:
+        n = self.n
+        o = Outputs(np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros((n, 3), np.float32),
+                    np.zeros(n, np.int32), np.zeros(n, np.int32))
+        _check(lib().tissue_get_outputs(self.h, _ptr(o.vx, C.c_float), _ptr(o.vy, C.c_float), _ptr(o.r3d, C.c_float),
+                                        _ptr(o.face, C.c_int32), _ptr(o.nbr, C.c_int32)))
+        return o
+
+    def step(self, n_steps: int, noise: np.ndarray | None = None) -> int:
+        """Run n_steps; returns the status code (validation errors are returned, not raised)."""
+        if noise is not None:
+            noise = np.ascontiguousarray(noise, np.float32)
+            assert noise.size >= n_steps * self.n
+        code = lib().tissue_step(self.h, n_steps, _ptr(noise, C.c_float))
+        if code not in (OK, E_LOST_PARTICLE, E_NONFINITE, E_TILING_NO_CONVERGE):
+            _check(code)
+        return code
+
+    def step_device(self, n_steps: int, noise_dev_ptr: int | None) -> int:
+        code = lib().tissue_step_device(self.h, n_steps, C.c_void_p(noise_dev_ptr) if noise_dev_ptr else None)
+        if code not in (OK, E_LOST_PARTICLE, E_NONFINITE, E_TILING_NO_CONVERGE):
+            _check(code)
+        return code
+
+    def stage_noise(self, noise: np.ndarray) -> None:
+        noise = np.ascontiguousarray(noise, np.float32).reshape(-1, self.n)
+        _check(lib().tissue_stage_noise(self.h, _ptr(noise, C.c_float), noise.shape[0]))
+
+    def step_staged(self, n_steps: int, first_row: int) -> int:
+        code = lib().tissue_step_staged(self.h, n_steps, first_row)
+        if code not in (OK, E_LOST_PARTICLE, E_NONFINITE, E_TILING_NO_CONVERGE):
+            _check(code)
+        return code
+
+    def order(self) -> tuple[np.ndarray, np.ndarray]:
+        cnt = C.c_int32()
+        _check(lib().tissue_get_order(self.h, None, None, 0, C.byref(cnt)))
+        a = np.zeros(cnt.value); b = np.zeros(cnt.value)
+        _check(lib().tissue_get_order(self.h, _ptr(a, C.c_double), _ptr(b, C.c_double), cnt.value, C.byref(cnt)))
+        return a, b
+
+    def reset_order(self) -> None:
+        _check(lib().tissue_reset_order(self.h))
+
+    def flags(self) -> tuple[int, int]:
+        f = C.c_uint32(); s = C.c_int32()
+        _check(lib().tissue_get_flags(self.h, C.byref(f), C.byref(s)))
+        return f.value, s.value
+
+    def neighbor_lists(self, cap: int = 64) -> list[np.ndarray]:
+        counts = np.zeros(self.n, np.int32); cols = np.zeros((self.n, cap), np.int32)
+        _check(lib().tissue_neighbor_lists(self.h, _ptr(counts, C.c_int32), _ptr(cols, C.c_int32), cap))
+        if counts.max(initial=0) > cap:
+            return self.neighbor_lists(int(counts.max()))
+        return [cols[i, : counts[i]].copy() for i in range(self.n)]
+
+    def set_launch(self, lanes: int = 0, block: int = 0) -> None:
+        _check(lib().tissue_set_launch(self.h, lanes, block))
+
+    def synchronize(self) -> None:
+        _check(lib().tissue_synchronize(self.h))
+
+    def last_step_ms(self) -> float:
+        ms = C.c_double()
+        _check(lib().tissue_last_step_ms(self.h, C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().tissue_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def gen_noise(seed: int, std: float, n_steps: int, n: int) -> np.ndarray:
+    out = np.zeros((n_steps, n), np.float32)
+    _check(lib().tissue_gen_noise(seed, std, n_steps, n, _ptr(out, C.c_float)))
+    return out
+
+
+def philox4x32(ctr, key) -> list[int]:
+    c = (C.c_uint32 * 4)(*ctr); k = (C.c_uint32 * 2)(*key); o = (C.c_uint32 * 4)()
+    lib().tissue_philox4x32(c, k, o)
+    return list(o)

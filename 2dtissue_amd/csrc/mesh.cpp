@@ -1,0 +1,644 @@
+// Mesh toolkit (host, FP64). See mesh.h for what each table replaces in the reference.
+#include "mesh.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <functional>
+#include <limits>
+#include <map>
+#include <numeric>
+#include <queue>
+#include <sstream>
+#include <stdexcept>
+#include <unordered_map>
+
+namespace tissue {
+
+namespace {
+
+inline double f32(double x) { return (double)(float)x; }
+
+inline void sub3(const double* a, const double* b, double* o) {
+  o[0] = a[0] - b[0]; o[1] = a[1] - b[1]; o[2] = a[2] - b[2];
+}
+inline double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline double norm3(const double* a) { return std::sqrt(dot3(a, a)); }
+inline void cross3(const double* a, const double* b, double* o) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ closed surfaces
+
+SurfaceMesh icosphere(int freq) {
+  if (freq < 1) throw std::runtime_error("icosphere: frequency must be >= 1");
+  const double phi = (1.0 + std::sqrt(5.0)) / 2.0;
+  std::vector<std::array<double, 3>> ico;
+  for (int s1 = -1; s1 <= 1; s1 += 2)
+    for (int s2 = -1; s2 <= 1; s2 += 2) {
+      ico.push_back({0.0, (double)s1, s2 * phi});
+      ico.push_back({(double)s1, s2 * phi, 0.0});
+      ico.push_back({s2 * phi, 0.0, (double)s1});
+    }
+  // faces of the icosahedron: triples of mutually adjacent corners (edge length 2)
+  std::vector<std::array<int, 3>> ifaces;
+  auto adj = [&](int i, int j) {
+    double d[3];
+    sub3(ico[i].data(), ico[j].data(), d);
+    return std::fabs(norm3(d) - 2.0) < 1e-9;
+  };
+  for (int i = 0; i < 12; ++i)
+    for (int j = i + 1; j < 12; ++j)
+      for (int k = j + 1; k < 12; ++k)
+        if (adj(i, j) && adj(j, k) && adj(i, k)) {
+          double e1[3], e2[3], n[3], c[3];
+          sub3(ico[j].data(), ico[i].data(), e1);
+          sub3(ico[k].data(), ico[i].data(), e2);
+          cross3(e1, e2, n);
+          for (int q = 0; q < 3; ++q) c[q] = ico[i][q] + ico[j][q] + ico[k][q];
+          if (dot3(n, c) > 0) ifaces.push_back({i, j, k});
+          else ifaces.push_back({i, k, j});
+        }
+  if (ifaces.size() != 20) throw std::runtime_error("icosphere: bad icosahedron");
+
+  SurfaceMesh m;
+  std::map<std::array<int, 6>, int> key2v;
+  auto vid = [&](const int* corner, const int* w) -> int {
+    // canonical key: (corner, weight) pairs with weight > 0, sorted by corner id
+    std::array<std::pair<int, int>, 3> cw;
+    int n = 0;
+    for (int q = 0; q < 3; ++q)
+      if (w[q] > 0) cw[n++] = {corner[q], w[q]};
+    std::sort(cw.begin(), cw.begin() + n);
+    std::array<int, 6> key{-1, -1, -1, -1, -1, -1};
+    for (int q = 0; q < n; ++q) { key[2 * q] = cw[q].first; key[2 * q + 1] = cw[q].second; }
+    auto it = key2v.find(key);
+    if (it != key2v.end()) return it->second;
+    double p[3] = {0, 0, 0};
+    for (int q = 0; q < 3; ++q)
+      for (int c = 0; c < 3; ++c) p[c] += (double)w[q] * ico[corner[q]][c];
+    double r = norm3(p);
+    int id = (int)(m.xyz.size() / 3);
+    for (int c = 0; c < 3; ++c) m.xyz.push_back(p[c] / r);
+    key2v.emplace(key, id);
+    return id;
+  };
+  for (const auto& f : ifaces) {
+    const int corner[3] = {f[0], f[1], f[2]};
+    auto P = [&](int i, int j) {
+      int w[3] = {freq - i - j, i, j};
+      return vid(corner, w);
+    };
+    for (int i = 0; i < freq; ++i)
+      for (int j = 0; i + j < freq; ++j) {
+        int a = P(i, j), b = P(i + 1, j), c = P(i, j + 1);
+        m.faces.insert(m.faces.end(), {a, b, c});
+        if (i + j <= freq - 2) {
+          int d = P(i + 1, j + 1);
+          m.faces.insert(m.faces.end(), {b, d, c});
+        }
+      }
+  }
+  if (m.nv() != 10 * freq * freq + 2 || m.nf() != 20 * freq * freq)
+    throw std::runtime_error("icosphere: unexpected vertex/face count");
+  return m;
+}
+
+void scale_axes(SurfaceMesh& m, double a, double b, double c) {
+  for (int v = 0; v < m.nv(); ++v) {
+    m.xyz[3 * v] *= a;
+    m.xyz[3 * v + 1] *= b;
+    m.xyz[3 * v + 2] *= c;
+  }
+}
+
+void perturb_y32(SurfaceMesh& m, double amp) {
+  // real orthonormal Y_3^2 = (1/4) sqrt(105/pi) sin^2(theta) cos(theta) cos(2 phi)
+  const double kY = 0.25 * std::sqrt(105.0 / M_PI);
+  for (int v = 0; v < m.nv(); ++v) {
+    double* p = &m.xyz[3 * v];
+    double r = norm3(p);
+    double ct = p[2] / r;
+    double st2 = std::max(0.0, 1.0 - ct * ct);
+    double ph = std::atan2(p[1], p[0]);
+    double rr = 1.0 + amp * kY * st2 * ct * std::cos(2.0 * ph);
+    for (int q = 0; q < 3; ++q) p[q] = p[q] / r * rr;
+  }
+}
+
+bool load_off(const std::string& path, SurfaceMesh& m, std::string& err) {
+  std::ifstream in(path);
+  if (!in) { err = "cannot open " + path; return false; }
+  std::vector<std::string> tok;
+  std::string line;
+  while (std::getline(in, line)) {
+    auto h = line.find('#');
+    if (h != std::string::npos) line = line.substr(0, h);
+    std::istringstream ls(line);
+    std::string t;
+    while (ls >> t) tok.push_back(t);
+  }
+  size_t p = 0;
+  if (tok.empty()) { err = "empty OFF file"; return false; }
+  if (tok[0] == "OFF") p = 1;
+  else if (tok[0].size() > 3 && tok[0].compare(0, 3, "OFF") == 0) { err = "unsupported OFF variant " + tok[0]; return false; }
+  if (tok.size() < p + 3) { err = "truncated OFF header"; return false; }
+  long nv = std::stol(tok[p]), nf = std::stol(tok[p + 1]);
+  p += 3;
+  if (nv <= 0 || nf <= 0) { err = "OFF file without vertices or faces"; return false; }
+  m.xyz.assign(3 * nv, 0.0);
+  m.faces.clear();
+  for (long v = 0; v < nv; ++v)
+    for (int c = 0; c < 3; ++c) {
+      if (p >= tok.size()) { err = "truncated OFF vertex list"; return false; }
+      m.xyz[3 * v + c] = std::stod(tok[p++]);
+    }
+  for (long f = 0; f < nf; ++f) {
+    if (p >= tok.size()) { err = "truncated OFF face list"; return false; }
+    long k = std::stol(tok[p++]);
+    if (k != 3) { err = "only triangle faces are supported"; return false; }
+    for (int c = 0; c < 3; ++c) {
+      long id = std::stol(tok[p++]);
+      if (id < 0 || id >= nv) { err = "face index out of range"; return false; }
+      m.faces.push_back((int32_t)id);
+    }
+  }
+  return true;
+}
+
+bool save_off(const std::string& path, const double* xyz, int nv, const int32_t* faces, int nf, int dim, std::string& err) {
+  FILE* f = std::fopen(path.c_str(), "w");
+  if (!f) { err = "cannot write " + path; return false; }
+  std::fprintf(f, "OFF\n%d %d 0\n", nv, nf);
+  for (int v = 0; v < nv; ++v) {
+    if (dim == 2) std::fprintf(f, "%.17g %.17g 0\n", xyz[2 * v], xyz[2 * v + 1]);
+    else std::fprintf(f, "%.17g %.17g %.17g\n", xyz[3 * v], xyz[3 * v + 1], xyz[3 * v + 2]);
+  }
+  for (int i = 0; i < nf; ++i) std::fprintf(f, "3 %d %d %d\n", faces[3 * i], faces[3 * i + 1], faces[3 * i + 2]);
+  std::fclose(f);
+  return true;
+}
+
+// ------------------------------------------------------------------ cut + UV map
+
+namespace {
+
+struct Graph {
+  std::vector<std::vector<std::pair<int, double>>> nbr;
+};
+
+Graph edge_graph(const std::vector<double>& xyz, const std::vector<int32_t>& faces, int nv) {
+  Graph g;
+  g.nbr.resize(nv);
+  auto add = [&](int a, int b) {
+    for (auto& e : g.nbr[a])
+      if (e.first == b) return;
+    double d[3];
+    sub3(&xyz[3 * a], &xyz[3 * b], d);
+    double L = norm3(d);
+    g.nbr[a].push_back({b, L});
+    g.nbr[b].push_back({a, L});
+  };
+  for (size_t f = 0; f < faces.size() / 3; ++f)
+    for (int q = 0; q < 3; ++q) add(faces[3 * f + q], faces[3 * f + (q + 1) % 3]);
+  return g;
+}
+
+std::vector<int> dijkstra_path(const Graph& g, int src, int dst, const std::vector<char>& blocked) {
+  int nv = (int)g.nbr.size();
+  std::vector<double> d(nv, std::numeric_limits<double>::infinity());
+  std::vector<int> prev(nv, -1);
+  using QE = std::pair<double, int>;
+  std::priority_queue<QE, std::vector<QE>, std::greater<QE>> pq;
+  d[src] = 0;
+  pq.push({0, src});
+  while (!pq.empty()) {
+    auto [dv, v] = pq.top();
+    pq.pop();
+    if (dv > d[v]) continue;
+    if (v == dst) break;
+    for (auto [w, L] : g.nbr[v]) {
+      if (blocked[w] && w != dst) continue;
+      if (dv + L < d[w]) { d[w] = dv + L; prev[w] = v; pq.push({d[w], w}); }
+    }
+  }
+  if (prev[dst] < 0 && src != dst) throw std::runtime_error("cut: no path between cone points");
+  std::vector<int> path;
+  for (int v = dst; v != -1; v = prev[v]) path.push_back(v);
+  std::reverse(path.begin(), path.end());
+  return path;
+}
+
+bool has_halfedge(const int32_t* f, int u, int w) {
+  for (int q = 0; q < 3; ++q)
+    if (f[q] == u && f[(q + 1) % 3] == w) return true;
+  return false;
+}
+
+// symmetric positive edge weights: mean-value weights (Floater 2003) averaged over both
+// directions, computed from 3D geometry
+std::vector<std::unordered_map<int, double>> mean_value_weights(const std::vector<double>& xyz,
+                                                                 const std::vector<int32_t>& faces, int nv) {
+  std::vector<std::unordered_map<int, double>> w(nv);
+  for (size_t f = 0; f < faces.size() / 3; ++f) {
+    for (int q = 0; q < 3; ++q) {
+      int i = faces[3 * f + q], j = faces[3 * f + (q + 1) % 3], l = faces[3 * f + (q + 2) % 3];
+      double eij[3], eil[3];
+      sub3(&xyz[3 * j], &xyz[3 * i], eij);
+      sub3(&xyz[3 * l], &xyz[3 * i], eil);
+      double lij = norm3(eij), lil = norm3(eil);
+      double c = std::clamp(dot3(eij, eil) / (lij * lil), -1.0, 1.0);
+      double t = std::tan(0.5 * std::acos(c));
+      w[i][j] += 0.5 * t / lij;   // half: averaged with the j->i weight below
+      w[j][i] += 0.5 * t / lij;
+      w[i][l] += 0.5 * t / lil;
+      w[l][i] += 0.5 * t / lil;
+    }
+  }
+  return w;
+}
+
+}  // namespace
+
+UVMesh cut_and_parameterize(const SurfaceMesh& m, int pa, int pm, int pb) {
+  const int V = m.nv(), F = m.nf();
+  if (V < 4 || F < 4) throw std::runtime_error("cut: mesh too small");
+  // Euler characteristic check (closed genus 0: V - E + F = 2)
+  Graph g = edge_graph(m.xyz, m.faces, V);
+  long E = 0;
+  for (auto& n : g.nbr) E += (long)n.size();
+  E /= 2;
+  if (V - E + F != 2) throw std::runtime_error("cut: mesh is not a closed genus-0 surface");
+
+  if (pa < 0 || pb < 0 || pm < 0) {
+    // cone points: the two extremes along x and the top in z
+    int ia = 0, ib = 0, im = 0;
+    for (int v = 1; v < V; ++v) {
+      if (m.xyz[3 * v] < m.xyz[3 * ia]) ia = v;
+      if (m.xyz[3 * v] > m.xyz[3 * ib]) ib = v;
+      if (m.xyz[3 * v + 2] > m.xyz[3 * im + 2]) im = v;
+    }
+    if (pa < 0) pa = ia;
+    if (pb < 0) pb = ib;
+    if (pm < 0) pm = im;
+  }
+  if (pa == pb || pa == pm || pb == pm || pa >= V || pb >= V || pm >= V)
+    throw std::runtime_error("cut: cone points must be three distinct vertices");
+
+  std::vector<char> blocked(V, 0);
+  std::vector<int> p1 = dijkstra_path(g, pa, pm, blocked);
+  for (int v : p1) blocked[v] = 1;
+  blocked[pm] = 0;
+  std::vector<int> p2 = dijkstra_path(g, pm, pb, blocked);
+  std::vector<int> path = p1;
+  path.insert(path.end(), p2.begin() + 1, p2.end());
+  const int K = (int)path.size() - 1;  // path = P0..PK, P0 = pa, PK = pb
+  const int m_idx = (int)p1.size() - 1;
+
+  // split the face fan of every interior path vertex into the two sides of the cut
+  std::vector<int32_t> faces = m.faces;
+  std::vector<std::vector<int>> vf(V);
+  for (int f = 0; f < F; ++f)
+    for (int q = 0; q < 3; ++q) vf[faces[3 * f + q]].push_back(f);
+  std::vector<int> dup(V, -1);
+  int nv = V;
+  std::vector<int32_t> orig(V);
+  std::iota(orig.begin(), orig.end(), 0);
+  for (int i = 1; i < K; ++i) {
+    const int v = path[i], prev = path[i - 1], next = path[i + 1];
+    const auto& fan = vf[v];
+    const int nfan = (int)fan.size();
+    std::vector<int> parent(nfan);
+    std::iota(parent.begin(), parent.end(), 0);
+    std::function<int(int)> find = [&](int x) { return parent[x] == x ? x : parent[x] = find(parent[x]); };
+    std::unordered_map<int, std::vector<int>> by_other;
+    for (int a = 0; a < nfan; ++a)
+      for (int q = 0; q < 3; ++q) {
+        int x = faces[3 * fan[a] + q];
+        if (x != v) by_other[x].push_back(a);
+      }
+    for (auto& [x, lst] : by_other) {
+      if (x == prev || x == next || (dup[prev] >= 0 && x == dup[prev])) continue;
+      if (lst.size() == 2) parent[find(lst[0])] = find(lst[1]);
+    }
+    int side_a = -1;
+    for (int a = 0; a < nfan; ++a)
+      if (has_halfedge(&faces[3 * fan[a]], prev, v)) side_a = find(a);
+    if (side_a < 0) throw std::runtime_error("cut: path edge without a forward face");
+    const int d = nv++;
+    dup[v] = d;
+    orig.push_back(v);
+    int nb = 0;
+    for (int a = 0; a < nfan; ++a)
+      if (find(a) != side_a) {
+        ++nb;
+        for (int q = 0; q < 3; ++q)
+          if (faces[3 * fan[a] + q] == v) faces[3 * fan[a] + q] = d;
+      }
+    if (nb == 0 || nb == nfan) throw std::runtime_error("cut: fan did not split in two");
+  }
+
+  UVMesh out;
+  out.nv = nv;
+  out.nf = F;
+  out.nv3d = V;
+  out.faces = faces;
+  out.orig = orig;
+  out.cut_path = path;
+  out.pa = pa;
+  out.pm = pm;
+  out.pb = pb;
+  out.xyz.resize(3 * nv);
+  for (int v = 0; v < nv; ++v)
+    for (int c = 0; c < 3; ++c) out.xyz[3 * v + c] = f32(m.xyz[3 * orig[v] + c]);
+
+  // boundary: arc-length parameter along Pa->Pm and Pm->Pb (3D edge lengths)
+  std::vector<double> uv(2 * nv, 0.0);
+  std::vector<char> fixed(nv, 0);
+  auto seglen = [&](int a, int b) {
+    double d[3];
+    sub3(&m.xyz[3 * a], &m.xyz[3 * b], d);
+    return norm3(d);
+  };
+  std::vector<double> s(K + 1, 0.0);
+  for (int i = 1; i <= K; ++i) s[i] = s[i - 1] + seglen(path[i - 1], path[i]);
+  const double L1 = s[m_idx], L2 = s[K] - s[m_idx];
+  for (int i = 0; i <= K; ++i) {
+    const int v = path[i];
+    double t;
+    if (i <= m_idx) {
+      t = f32(s[i] / L1);
+      if (i == 0) t = 0.0;
+      if (i == m_idx) t = 1.0;
+      uv[2 * v] = t;      // side A: down edge (t, 0)
+      uv[2 * v + 1] = 0.0;
+      if (dup[v] >= 0) { uv[2 * dup[v]] = 0.0; uv[2 * dup[v] + 1] = t; }  // side B: left edge (0, t)
+    } else {
+      t = f32((s[i] - s[m_idx]) / L2);
+      if (i == K) t = 1.0;
+      uv[2 * v] = 1.0;    // side A: right edge (1, t)
+      uv[2 * v + 1] = t;
+      if (dup[v] >= 0) { uv[2 * dup[v]] = t; uv[2 * dup[v] + 1] = 1.0; }  // side B: up edge (t, 1)
+    }
+    fixed[v] = 1;
+    if (dup[v] >= 0) fixed[dup[v]] = 1;
+  }
+
+  // interior: harmonic map with symmetric mean-value weights, solved by Jacobi-preconditioned CG
+  auto w = mean_value_weights(out.xyz, faces, nv);
+  std::vector<int> idx(nv, -1);
+  int ni = 0;
+  for (int v = 0; v < nv; ++v)
+    if (!fixed[v]) idx[v] = ni++;
+  std::vector<int> rp(ni + 1, 0), ci;
+  std::vector<double> val, diag(ni, 0.0), bx(ni, 0.0), by(ni, 0.0);
+  for (int v = 0; v < nv; ++v) {
+    if (fixed[v]) continue;
+    const int r = idx[v];
+    std::vector<std::pair<int, double>> row(w[v].begin(), w[v].end());
+    std::sort(row.begin(), row.end());
+    for (auto [u, wt] : row) {
+      diag[r] += wt;
+      if (fixed[u]) { bx[r] += wt * uv[2 * u]; by[r] += wt * uv[2 * u + 1]; }
+      else { ci.push_back(idx[u]); val.push_back(-wt); }
+    }
+    rp[r + 1] = (int)ci.size();
+  }
+  auto solve = [&](const std::vector<double>& b, std::vector<double>& x) {
+    x.assign(ni, 0.0);
+    std::vector<double> r = b, z(ni), p(ni), Ap(ni);
+    for (int i = 0; i < ni; ++i) { z[i] = r[i] / diag[i]; p[i] = z[i]; }
+    double rz = 0, bn = 0;
+    for (int i = 0; i < ni; ++i) { rz += r[i] * z[i]; bn += b[i] * b[i]; }
+    bn = std::sqrt(bn);
+    if (bn == 0) return;
+    for (int it = 0; it < 20 * ni + 100; ++it) {
+      for (int i = 0; i < ni; ++i) {
+        double a = diag[i] * p[i];
+        for (int e = rp[i]; e < rp[i + 1]; ++e) a += val[e] * p[ci[e]];
+        Ap[i] = a;
+      }
+      double pAp = 0;
+      for (int i = 0; i < ni; ++i) pAp += p[i] * Ap[i];
+      double alpha = rz / pAp, rn = 0;
+      for (int i = 0; i < ni; ++i) { x[i] += alpha * p[i]; r[i] -= alpha * Ap[i]; rn += r[i] * r[i]; }
+      if (std::sqrt(rn) <= 1e-13 * bn) return;
+      double rz2 = 0;
+      for (int i = 0; i < ni; ++i) { z[i] = r[i] / diag[i]; rz2 += r[i] * z[i]; }
+      double beta = rz2 / rz;
+      rz = rz2;
+      for (int i = 0; i < ni; ++i) p[i] = z[i] + beta * p[i];
+    }
+    throw std::runtime_error("cut: UV solve did not converge");
+  };
+  std::vector<double> sx, sy;
+  solve(bx, sx);
+  solve(by, sy);
+  for (int v = 0; v < nv; ++v)
+    if (!fixed[v]) {
+      uv[2 * v] = f32(std::clamp(sx[idx[v]], 0.0, 1.0));
+      uv[2 * v + 1] = f32(std::clamp(sy[idx[v]], 0.0, 1.0));
+    }
+
+  // orientation: every UV triangle must have the same, nonzero orientation; swapping u and v
+  // flips it and maps the gluing (t,0)~(0,t), (1,t)~(t,1) onto itself
+  int npos = 0, nneg = 0;
+  for (int f = 0; f < F; ++f) {
+    const double* a = &uv[2 * faces[3 * f]];
+    const double* b = &uv[2 * faces[3 * f + 1]];
+    const double* c = &uv[2 * faces[3 * f + 2]];
+    double ar = (b[0] - a[0]) * (c[1] - a[1]) - (b[1] - a[1]) * (c[0] - a[0]);
+    if (ar > 0) ++npos;
+    else if (ar < 0) ++nneg;
+  }
+  if (npos != F && nneg != F) {
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "cut: UV map folds (%d positive, %d negative, %d degenerate triangles)", npos,
+                  nneg, F - npos - nneg);
+    throw std::runtime_error(buf);
+  }
+  if (nneg == F)
+    for (int v = 0; v < nv; ++v) std::swap(uv[2 * v], uv[2 * v + 1]);
+  out.uv = uv;
+  return out;
+}
+
+// ------------------------------------------------------------------ face buckets
+
+void build_face_buckets(UVMesh& m, int grid) {
+  if (grid <= 0) grid = std::max(1, (int)std::ceil(std::sqrt(1.0 * m.nf)));
+  m.grid = grid;
+  const double eps = 1e-9;
+  std::vector<std::vector<int32_t>> cells((size_t)grid * grid);
+  for (int f = 0; f < m.nf; ++f) {
+    double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300};
+    for (int q = 0; q < 3; ++q)
+      for (int c = 0; c < 2; ++c) {
+        double x = m.uv[2 * m.faces[3 * f + q] + c];
+        lo[c] = std::min(lo[c], x);
+        hi[c] = std::max(hi[c], x);
+      }
+    int c0[2], c1[2];
+    for (int c = 0; c < 2; ++c) {
+      c0[c] = std::clamp((int)std::floor((lo[c] - eps) * grid), 0, grid - 1);
+      c1[c] = std::clamp((int)std::floor((hi[c] + eps) * grid), 0, grid - 1);
+    }
+    for (int cy = c0[1]; cy <= c1[1]; ++cy)
+      for (int cx = c0[0]; cx <= c1[0]; ++cx) cells[(size_t)cy * grid + cx].push_back(f);
+  }
+  m.cell_start.assign((size_t)grid * grid + 1, 0);
+  m.cell_faces.clear();
+  for (size_t c = 0; c < cells.size(); ++c) {
+    m.cell_faces.insert(m.cell_faces.end(), cells[c].begin(), cells[c].end());
+    m.cell_start[c + 1] = (int32_t)m.cell_faces.size();
+  }
+}
+
+// ------------------------------------------------------------------ geodesic table
+
+namespace {
+
+// first-order fast-marching update of vertex C from triangle (A, B, C) with known arrival
+// times tA, tB (Kimmel & Sethian 1998, written for 3D edge vectors): the planar front
+// T(P) = tC + g.(P - C), |g| = 1, through A and B; accepted only when the characteristic
+// reaching C enters the triangle through edge AB.
+double tri_update(const double* A, double tA, const double* B, double tB, const double* C) {
+  double e1[3], e2[3];
+  sub3(A, C, e1);
+  sub3(B, C, e2);
+  const double m00 = dot3(e1, e1), m01 = dot3(e1, e2), m11 = dot3(e2, e2);
+  const double det = m00 * m11 - m01 * m01;
+  if (!(det > 1e-300)) return std::numeric_limits<double>::infinity();
+  const double i00 = m11 / det, i01 = -m01 / det, i11 = m00 / det;
+  const double a = i00 + 2 * i01 + i11;
+  const double b = (i00 + i01) * tA + (i01 + i11) * tB;
+  const double c = i00 * tA * tA + 2 * i01 * tA * tB + i11 * tB * tB - 1.0;
+  const double disc = b * b - a * c;
+  if (!(a > 0) || disc < 0) return std::numeric_limits<double>::infinity();
+  const double T = (b + std::sqrt(disc)) / a;
+  const double r0 = tA - T, r1 = tB - T;
+  const double c0 = i00 * r0 + i01 * r1, c1 = i01 * r0 + i11 * r1;
+  const double tol = 1e-12;
+  if (c0 > tol || c1 > tol || T < std::max(tA, tB)) return std::numeric_limits<double>::infinity();
+  return T;
+}
+
+// length of the straight path from V to X across edge (A, B) once the two faces (V,A,B) and
+// (A,B,X) are unfolded into one plane; infinity when that segment misses the edge
+double unfold_across(const double* V, const double* A, const double* B, const double* X) {
+  double ab[3], av[3], ax[3];
+  sub3(B, A, ab);
+  sub3(V, A, av);
+  sub3(X, A, ax);
+  const double L = norm3(ab);
+  if (!(L > 0)) return std::numeric_limits<double>::infinity();
+  const double vu = dot3(av, ab) / L, xu = dot3(ax, ab) / L;
+  const double vh = std::sqrt(std::max(0.0, dot3(av, av) - vu * vu));
+  const double xh = std::sqrt(std::max(0.0, dot3(ax, ax) - xu * xu));
+  if (!(vh + xh > 0)) return std::numeric_limits<double>::infinity();
+  const double t = vu + (xu - vu) * vh / (vh + xh);   // crossing point along AB
+  if (t < 0 || t > L) return std::numeric_limits<double>::infinity();
+  return std::sqrt((xu - vu) * (xu - vu) + (xh + vh) * (xh + vh));
+}
+
+}  // namespace
+
+void build_distance_table(UVMesh& m, double cutoff) {
+  if (!(cutoff > 0)) throw std::runtime_error("distance table: cutoff must be positive");
+  const int nv = m.nv;
+  const double lim = cutoff * (1.0 + 1e-6);
+  std::vector<std::vector<int>> vf(nv);
+  for (int f = 0; f < m.nf; ++f)
+    for (int q = 0; q < 3; ++q) vf[m.faces[3 * f + q]].push_back(f);
+
+  std::unordered_map<int64_t, std::vector<int>> ef;
+  auto edge_key = [](int a, int b) { return a < b ? ((int64_t)a << 32) | b : ((int64_t)b << 32) | a; };
+  for (int f = 0; f < m.nf; ++f)
+    for (int q = 0; q < 3; ++q) ef[edge_key(m.faces[3 * f + q], m.faces[3 * f + (q + 1) % 3])].push_back(f);
+  std::vector<double> T(nv, std::numeric_limits<double>::infinity());
+  std::vector<char> alive(nv, 0);
+  std::vector<int> touched;
+  std::vector<std::vector<std::pair<int, double>>> rows(nv);
+  using QE = std::pair<double, int>;
+  for (int s = 0; s < nv; ++s) {
+    std::priority_queue<QE, std::vector<QE>, std::greater<QE>> pq;
+    auto relax = [&](int tgt, double cand) {
+      if (cand < T[tgt]) {
+        if (T[tgt] == std::numeric_limits<double>::infinity()) touched.push_back(tgt);
+        T[tgt] = cand;
+        pq.push({cand, tgt});
+      }
+    };
+    T[s] = 0;
+    touched.push_back(s);
+    pq.push({0.0, s});
+    while (!pq.empty()) {
+      auto [tv, v] = pq.top();
+      pq.pop();
+      if (alive[v] || tv > T[v]) continue;
+      if (tv > lim) break;
+      alive[v] = 1;
+      rows[s].push_back({v, tv});
+      const double* X = &m.xyz[3 * v];
+      for (int f : vf[v]) {
+        const int32_t* fc = &m.faces[3 * f];
+        for (int q = 0; q < 3; ++q) {
+          const int tgt = fc[q];
+          if (tgt == v || alive[tgt]) continue;
+          const int oth = fc[0] + fc[1] + fc[2] - v - tgt;
+          double d[3];
+          sub3(&m.xyz[3 * tgt], X, d);
+          double cand = tv + norm3(d);
+          if (alive[oth]) cand = std::min(cand, tri_update(X, tv, &m.xyz[3 * oth], T[oth], &m.xyz[3 * tgt]));
+          relax(tgt, cand);
+        }
+        // straight segment from v across the opposite edge (a, b) into the neighbouring face
+        const int a = fc[0] == v ? fc[1] : fc[0];
+        const int b = fc[0] + fc[1] + fc[2] - v - a;
+        for (int g2 : ef[edge_key(a, b)]) {
+          if (g2 == f) continue;
+          const int32_t* gc = &m.faces[3 * g2];
+          const int x = gc[0] + gc[1] + gc[2] - a - b;
+          if (alive[x]) continue;
+          double L = unfold_across(X, &m.xyz[3 * a], &m.xyz[3 * b], &m.xyz[3 * x]);
+          relax(x, tv + L);
+        }
+      }
+    }
+    for (int v : touched) { T[v] = std::numeric_limits<double>::infinity(); alive[v] = 0; }
+    touched.clear();
+  }
+  // symmetrize with the minimum of both directions (transform_into_symmetric_matrix,
+  // src/simulation/Locomotion.cpp:127-138) and round to FP32
+  std::vector<std::unordered_map<int, double>> sym(nv);
+  for (int s = 0; s < nv; ++s)
+    for (auto [v, t] : rows[s]) {
+      auto upd = [&](int a, int b) {
+        auto it = sym[a].find(b);
+        if (it == sym[a].end()) sym[a].emplace(b, t);
+        else it->second = std::min(it->second, t);
+      };
+      upd(s, v);
+      upd(v, s);
+    }
+  m.cutoff = cutoff;
+  m.row_ptr.assign(nv + 1, 0);
+  m.col.clear();
+  m.dist.clear();
+  for (int v = 0; v < nv; ++v) {
+    std::vector<std::pair<float, int>> r;
+    r.reserve(sym[v].size());
+    for (auto [w, t] : sym[v]) r.push_back({(float)(w == v ? 0.0 : t), w});
+    std::sort(r.begin(), r.end());
+    for (auto [d, w] : r) { m.col.push_back(w); m.dist.push_back(d); }
+    m.row_ptr[v + 1] = (int32_t)m.col.size();
+  }
+}
+
+}  // namespace tissue

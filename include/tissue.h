@@ -1,0 +1,124 @@
+/* tissue.h - C-ABI of the MI355X (gfx950) per-step particle update of 2DTissue.
+ *
+ * The reference has no FFI: its boundary is the C++ class _2DTissue (src/simulation/2DTissue.h:34-129)
+ * and the two strategy interfaces LocomotionHelperInterface / OrientationHelperInterface
+ * (src/simulation/LocomotionHelperInterface.h:3-13). This header is the plain-C layer under a
+ * drop-in replacement of that class (include/2DTissue.h); every entry point below names the
+ * reference code it replaces.
+ *
+ * Conventions
+ *  - Every call returns an int32 status (TISSUE_OK = 0). No exceptions cross the boundary.
+ *    tissue_error_string() returns the calling thread's last message.
+ *  - Host pointers are borrowed for the duration of a call only. "_device" entry points take
+ *    device pointers (memory on the simulation's GPU).
+ *  - A tissue_sim / tissue_batch owns its device memory and one HIP stream (default priority).
+ *    A handle is not thread-safe; distinct handles are independent. A mesh may be shared by any
+ *    number of handles and must outlive them.
+ *  - Arrays are structure-of-arrays, FP32 for state, int32 for ids; r3d is n*3 row-major.
+ */
+#ifndef TISSUE_H
+#define TISSUE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define TISSUE_OK 0
+#define TISSUE_E_INVALID_ARG 1
+#define TISSUE_E_LOST_PARTICLE 2      /* Validation::error_lost_particles throws   (Validation.cpp:66-72)  */
+#define TISSUE_E_NONFINITE 3          /* Validation::error_invalid_values exits     (Validation.cpp:40-46)  */
+#define TISSUE_E_TILING_NO_CONVERGE 4 /* seam wrap over its crossing bound; the reference loops forever
+                                         (Locomotion/EuclideanTiling.cpp:41-68)                          */
+#define TISSUE_E_HIP 5                /* HIP runtime error (no device, launch failure, out of memory)      */
+#define TISSUE_E_IO 6
+#define TISSUE_E_MESH 7               /* mesh is not a closed genus-0 surface, or its UV map folds         */
+
+/* flag bits reported by tissue_get_flags (a step may raise several) */
+#define TISSUE_FLAG_LOST_PARTICLE 1u
+#define TISSUE_FLAG_NONFINITE 2u
+#define TISSUE_FLAG_TILING 4u
+
+typedef struct tissue_mesh tissue_mesh;
+typedef struct tissue_sim tissue_sim;
+
+/* Physics of one simulation. Replaces the _2DTissue constructor arguments (2DTissue.h:37-54).
+ * speed = v0 + mu |F|; heading += (dt/tau) wrap(mean - heading) + eta. With mu = 1 and tau = dt
+ * this is the reference's update (which has neither parameter). */
+typedef struct {
+  double sigma;          /* particle radius (2DTissue.h:49)                                   */
+  double v0;             /* self-propulsion speed (2DTissue.h:44)                             */
+  double k;              /* repulsion stiffness (2DTissue.h:46)                               */
+  double mu;             /* mobility (2DTissue.h:50)                                          */
+  double tau;            /* alignment time                                                   */
+  double dt;             /* step size (2DTissue.h:53, step_size)                               */
+  int32_t max_crossings; /* seam crossings allowed per particle per step; <= 0 selects 16384   */
+  int32_t reserved;
+} tissue_params;
+
+const char* tissue_error_string(void);
+
+/* ---- mesh: replaces the construction-time MeshCartographyLib calls (2DTissue.cpp:85-107) ---- */
+/* icosphere of frequency freq (V = 10 freq^2 + 2) scaled to semi-axes (a, b, c) */
+int32_t tissue_mesh_ellipsoid(double a, double b, double c, int32_t freq, tissue_mesh** out);
+/* unit icosphere with radius 1 + amp * Y_3^2(theta, phi) */
+int32_t tissue_mesh_perturbed_sphere(double amp, int32_t freq, tissue_mesh** out);
+/* closed genus-0 triangle mesh from an OFF file (loadMeshFaces + create_uv_surface, 2DTissue.cpp:85-91) */
+int32_t tissue_mesh_load_off(const char* path, tissue_mesh** out);
+/* face buckets (grid <= 0: automatic) and the geodesic table on the open mesh, truncated at
+ * 2.4 * sigma_max (CachedGeodesicDistanceHelper(<stem>_open.off), 2DTissue.cpp:102-105) */
+int32_t tissue_mesh_build_tables(tissue_mesh* m, double sigma_max, int32_t grid);
+void tissue_mesh_destroy(tissue_mesh* m);
+int32_t tissue_mesh_info(const tissue_mesh* m, int32_t* nv, int32_t* nf, int32_t* nv3d, int64_t* nnz, int32_t* grid,
+                         int64_t* n_cell_faces);
+/* copies of the tables the step reads; any NULL pointer is skipped */
+int32_t tissue_mesh_export(const tissue_mesh* m, double* uv /* nv*2 */, double* xyz /* nv*3 */, int32_t* faces,
+                           int32_t* row_ptr, int32_t* col, float* dist, int32_t* cell_start, int32_t* cell_faces);
+/* the 3-D mesh and its UV chart as OFF files (loadMeshFaces' format) */
+int32_t tissue_mesh_save_off(const tissue_mesh* m, const char* path_3d, const char* path_uv);
+
+/* ---- one simulation on the current GPU ---- */
+/* replaces _2DTissue::_2DTissue (2DTissue.cpp:20-111) for the per-step state */
+int32_t tissue_create(const tissue_mesh* m, const tissue_params* p, int32_t n, tissue_sim** out);
+void tissue_destroy(tissue_sim* s);
+/* replaces _2DTissue::start (2DTissue.cpp:117-134): positions uniform by area, headings uniform
+ * on [0, 2 pi), then the UV->3D remap for the vertex ids */
+int32_t tissue_init_random(tissue_sim* s, uint64_t seed);
+/* state upload; vid == NULL computes vertex ids with the remap, as start() does */
+int32_t tissue_set_state(tissue_sim* s, const float* x, const float* y, const float* theta, const int32_t* vid);
+int32_t tissue_get_state(tissue_sim* s, float* x, float* y, float* theta, int32_t* vid);
+/* per-particle outputs of the last step (System/Particle, Struct.h:9-29; 2DTissue.cpp:148-166) */
+int32_t tissue_get_outputs(tissue_sim* s, float* vx, float* vy, float* r3d, int32_t* face, int32_t* nbr);
+/* replaces n_steps calls of _2DTissue::update (2DTissue.cpp:136-206): noise is eta per step and
+ * particle, host memory [n_steps][n] (NULL: zero, as the reference's uniform(0,0) noise) */
+int32_t tissue_step(tissue_sim* s, int32_t n_steps, const float* noise);
+int32_t tissue_step_device(tissue_sim* s, int32_t n_steps, const float* noise_device);
+/* copy rows [n_rows][n] of eta into the handle's device buffer once, then step from row first_row */
+int32_t tissue_stage_noise(tissue_sim* s, const float* noise, int32_t n_rows);
+int32_t tissue_step_staged(tissue_sim* s, int32_t n_steps, int32_t first_row);
+/* order parameter history (v_order, 2DTissue.cpp:242; get_order_parameter, :213-216):
+ * phi_ref = reference reading, phi_polar = |sum (cos, sin)| / n; either pointer may be NULL */
+int32_t tissue_get_order(tissue_sim* s, double* phi_ref, double* phi_polar, int32_t cap, int32_t* count);
+int32_t tissue_reset_order(tissue_sim* s);
+/* validation flags raised since the last call (clears them) and the first step that raised one */
+int32_t tissue_get_flags(tissue_sim* s, uint32_t* flags, int32_t* first_step);
+/* neighbour sets {j : d_ij < 2 sigma} (self included) of the current state, at most cap per
+ * particle: counts[n], cols[n*cap], each list ascending */
+int32_t tissue_neighbor_lists(tissue_sim* s, int32_t* counts, int32_t* cols, int32_t cap);
+/* launch shape: lanes per particle (1, 2, 4, 8, 16) and threads per block (64..1024); 0 = auto */
+int32_t tissue_set_launch(tissue_sim* s, int32_t lanes, int32_t block);
+int32_t tissue_synchronize(tissue_sim* s);
+/* GPU time of the step kernels of the last tissue_step call (HIP events on the handle's stream) */
+int32_t tissue_last_step_ms(tissue_sim* s, double* ms);
+
+/* ---- host helpers (no GPU) ---- */
+/* eta[t][i] = std * N(0,1) from Philox4x32-10 (Salmon et al., SC'11) + Box-Muller */
+int32_t tissue_gen_noise(uint64_t seed, double std, int32_t n_steps, int32_t n, float* out);
+/* Philox4x32-10 block for (counter, key); exposed for known-answer tests */
+void tissue_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

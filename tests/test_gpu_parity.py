@@ -1,0 +1,42 @@
+"""GPU parity: the HIP step (through the C-ABI) against the FP64 oracle, teacher-forced: every
+step of the GPU trajectory is recomputed by the oracle from the GPU's own input state."""
+import numpy as np
+import pytest
+
+import parity as P
+
+pytestmark = pytest.mark.gpu
+
+
+def run_teacher_forced(T, O, cfg, steps, lanes=0):
+    m, tb = P.mesh_and_tables(cfg["freq"], cfg["sigma"])
+    om = O.Mesh(tb)
+    p = dict(P.PHYS, sigma=cfg["sigma"])
+    sim = T.Sim(m, T.params(**p), cfg["n"])
+    sim.set_launch(lanes, 0)
+    sim.init_random(cfg["seed"])
+    noise = T.gen_noise(cfg["noise_seed"], 0.1, steps, cfg["n"])
+    total = {}
+    for t in range(steps):
+        s_in = sim.get_state()
+        nb = sim.neighbor_lists()
+        assert sim.step(1, noise[t]) == T.OK
+        s_out = sim.get_state()
+        out = sim.get_outputs()
+        fails, _ = P.compare_step(O, om, p, s_in, s_out, out, nb, noise[t].astype(np.float64))
+        for k, v in fails.items():
+            total[k] = total.get(k, 0) + v
+        if any(fails.values()):
+            print(f"step {t}: {fails}", flush=True)
+    return total
+
+
+@pytest.mark.parametrize("lanes", [1, 8])
+def test_c1_teacher_forced_50_steps(T, O, lanes):
+    total = run_teacher_forced(T, O, P.C1, 50, lanes)
+    assert all(v == 0 for v in total.values()), total
+
+
+def test_c2_teacher_forced_20_steps(T, O):
+    total = run_teacher_forced(T, O, P.C2, 20)
+    assert all(v == 0 for v in total.values()), total

@@ -1,0 +1,178 @@
+"""CPU tests of the product library: it loads without a GPU and exports every entry point of
+include/tissue.h; the host RNG; the mesh toolkit (UV chart with the reference's gluing, face
+buckets, geodesic table). No GPU compute is called here."""
+import math
+import os
+import re
+import tempfile
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    src = open(os.path.join(ROOT, "include", "tissue.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(tissue_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol(T):
+    names = header_functions()
+    assert len(names) >= 25
+    lib = T.lib()
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    bound = {n for n, _, _ in T.SIGNATURES}
+    assert set(names) <= bound, set(names) - bound
+
+
+def test_philox_known_answers(T):
+    """Philox4x32-10 known-answer vectors of Random123 (Salmon et al., SC'11)."""
+    assert T.philox4x32([0, 0, 0, 0], [0, 0]) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert T.philox4x32([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2) == [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]
+    assert T.philox4x32([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0]) == \
+        [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+def test_noise_statistics_and_determinism(T):
+    a = T.gen_noise(7, 0.1, 200, 1000)
+    assert np.array_equal(a, T.gen_noise(7, 0.1, 200, 1000))
+    assert not np.array_equal(a, T.gen_noise(8, 0.1, 200, 1000))
+    assert abs(a.mean()) < 0.1 * 4 / math.sqrt(a.size)
+    assert abs(a.std() - 0.1) < 0.1 * 0.01
+    # counter-based: a prefix of particles/steps does not depend on the array shape
+    assert np.array_equal(T.gen_noise(7, 0.1, 3, 10), a[:3, :10])
+
+
+@pytest.fixture(scope="module")
+def ell(T):
+    m = T.Mesh.ellipsoid(2.0, 1.0, 1.0, 12).build_tables(0.1)
+    return m, m.export()
+
+
+def test_mesh_counts_and_uv_chart(ell):
+    m, t = ell
+    f = 12
+    assert t["nv3d"] == 10 * f * f + 2 and t["nf"] == 20 * f * f
+    uv, faces = t["uv"], t["faces"]
+    assert uv.min() >= 0 and uv.max() <= 1
+    assert np.array_equal(uv, uv.astype(np.float32).astype(np.float64))  # FP32-representable
+    a, b, c = uv[faces[:, 0]], uv[faces[:, 1]], uv[faces[:, 2]]
+    area = 0.5 * ((b[:, 0] - a[:, 0]) * (c[:, 1] - a[:, 1]) - (b[:, 1] - a[:, 1]) * (c[:, 0] - a[:, 0]))
+    assert (area > 0).all()                     # counter-clockwise, no folds
+    assert abs(area.sum() - 1.0) < 1e-9         # the triangles tile the unit square
+
+
+def test_mesh_seam_gluing(ell):
+    """Seam duplicates sit where the seam wrap sends their twin: (t,0)~(0,t) and (1,t)~(t,1)."""
+    m, t = ell
+    uv, xyz = t["uv"], t["xyz"]
+    nv3d, nv = t["nv3d"], t["nv"]
+    key = {tuple(xyz[v]): v for v in range(nv3d)}
+    n_dup = 0
+    for d in range(nv3d, nv):
+        v = key[tuple(xyz[d])]
+        (x0, y0), (x1, y1) = uv[v], uv[d]
+        if y0 == 0.0:
+            assert (x1, y1) == (0.0, x0)
+        else:
+            assert x0 == 1.0 and (x1, y1) == (y0, 1.0)
+        n_dup += 1
+    assert n_dup == nv - nv3d > 0
+    # the boundary of the chart is exactly the unit square's edges
+    edges = {}
+    for f in t["faces"]:
+        for q in range(3):
+            e = tuple(sorted((f[q], f[(q + 1) % 3])))
+            edges[e] = edges.get(e, 0) + 1
+    bverts = {v for e, c in edges.items() if c == 1 for v in e}
+    for v in bverts:
+        x, y = uv[v]
+        assert x in (0.0, 1.0) or y in (0.0, 1.0)
+
+
+def test_face_buckets_cover_faces(ell):
+    m, t = ell
+    g, cs, cf = t["grid"], t["cell_start"], t["cell_faces"]
+    uv, faces = t["uv"], t["faces"]
+    cen = uv[faces].mean(1)
+    cells = np.minimum((cen * g).astype(int), g - 1)
+    for fi, (cx, cy) in enumerate(cells):
+        c = cy * g + cx
+        assert fi in cf[cs[c]:cs[c + 1]]
+
+
+def test_distance_table_structure(ell):
+    m, t = ell
+    rp, col, dist = t["row_ptr"], t["col"], t["dist"]
+    nv = t["nv"]
+    lookup = {}
+    for v in range(nv):
+        row = list(zip(dist[rp[v]:rp[v + 1]], col[rp[v]:rp[v + 1]]))
+        assert row == sorted(row)               # ascending (distance, vertex)
+        assert (0.0, v) in row                  # diagonal present, zero
+        for d, w in row:
+            lookup[(v, w)] = d
+            assert d <= 2.4 * 0.1 * (1 + 1e-6)
+    for (v, w), d in lookup.items():
+        assert lookup[(w, v)] == d              # symmetric (min of both directions)
+
+
+def test_geodesic_table_vs_chord_and_sphere(T):
+    """On the unit sphere the table must lie between the chord (a lower bound) and the
+    great-circle distance plus the mesh's discretisation error."""
+    m = T.Mesh.ellipsoid(1.0, 1.0, 1.0, 16).build_tables(0.15)
+    t = m.export()
+    xyz, rp, col, dist = t["xyz"], t["row_ptr"], t["col"], t["dist"]
+    ratios = []
+    for v in range(0, t["nv"], 7):
+        for e in range(rp[v], rp[v + 1]):
+            w = col[e]
+            ch = np.linalg.norm(xyz[v] - xyz[w])
+            if w == v or ch == 0 or dist[e] > 0.3:
+                continue
+            gc = 2 * math.asin(min(1.0, ch / 2))
+            assert dist[e] >= ch * (1 - 1e-6)
+            ratios.append(dist[e] / gc)
+    r = np.array(ratios)
+    assert np.median(r) < 1.01 and np.percentile(r, 99) < 1.08
+
+
+def test_off_roundtrip(T, ell):
+    m, t = ell
+    with tempfile.TemporaryDirectory() as d:
+        p3, puv = os.path.join(d, "e.off"), os.path.join(d, "e_uv.off")
+        m.save_off(p3, puv)
+        assert open(puv).readline().strip() == "OFF"
+        with pytest.raises(T.TissueError):
+            T.Mesh.load_off(p3)  # the UV mesh's duplicated seam vertices make it open, not genus 0
+        src = os.path.join(d, "closed.off")  # a closed icosahedron written by hand
+        phi = (1 + 5 ** 0.5) / 2
+        V = [(0, s1, s2 * phi) for s1 in (-1, 1) for s2 in (-1, 1)] + \
+            [(s1, s2 * phi, 0) for s1 in (-1, 1) for s2 in (-1, 1)] + \
+            [(s2 * phi, 0, s1) for s1 in (-1, 1) for s2 in (-1, 1)]
+        V = np.array(V, float)
+        F = []
+        for i in range(12):
+            for j in range(i + 1, 12):
+                for k in range(j + 1, 12):
+                    if all(abs(np.linalg.norm(V[a] - V[b]) - 2) < 1e-9 for a, b in ((i, j), (j, k), (i, k))):
+                        n = np.cross(V[j] - V[i], V[k] - V[i])
+                        F.append((i, j, k) if n @ (V[i] + V[j] + V[k]) > 0 else (i, k, j))
+        with open(src, "w") as fh:
+            fh.write(f"OFF\n12 {len(F)} 0\n")
+            for v in V:
+                fh.write("%.17g %.17g %.17g\n" % tuple(v))
+            for f in F:
+                fh.write("3 %d %d %d\n" % f)
+        mm = T.Mesh.load_off(src)
+        assert mm.info()["nv3d"] == 12 and mm.info()["nf"] == 20
+
+
+def test_mesh_errors(T):
+    with pytest.raises(T.TissueError):
+        T.Mesh.ellipsoid(-1.0, 1.0, 1.0, 4)
+    with pytest.raises(T.TissueError):
+        T.Mesh.load_off("/nonexistent/mesh.off")

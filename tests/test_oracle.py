@@ -1,0 +1,299 @@
+"""Pin the FP64 oracle: the reference's own test vectors (tests/golden/reference_vectors.json),
+then closed forms, brute force and invariants for what those vectors leave open. Each check is
+chosen so that a plausible slip in the restatement (a dropped term, a wrong sign or index, a
+transposed operand) fails it."""
+import ctypes as C
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.json")))
+
+
+def test_angles_to_unit_vectors(O):
+    g = GOLD["angles_to_unit_vectors"]
+    n = np.array(g["n_deg"], np.int32)
+    out = np.zeros((len(n), 2))
+    O.lib().or_angles_to_unit_vectors(O.ip(n), len(n), O.dp(out))
+    assert np.abs(out - np.array(g["expected"])).max() <= g["tol"]
+
+
+def test_seam_wrap_reference_vector(O):
+    g = GOLD["seam_wrap"]
+    st, r, _, h, cr = O.seam_wrap(g["r"], g["r_old"], g["n"], unit_deg=1)
+    assert st == O.OK
+    assert np.abs(r - np.array(g["expected_r"])).max() <= g["tol"]
+    # headings (not asserted by the reference test): particle 0 crosses right then down
+    assert list(h) == [-280.0, -60.0, -48.0]
+    assert list(cr) == [2, 2, 1]
+
+
+def test_seam_wrap_radians_matches_degrees(O):
+    g = GOLD["seam_wrap"]
+    _, r_d, _, h_d, _ = O.seam_wrap(g["r"], g["r_old"], g["n"], unit_deg=1)
+    _, r_r, _, h_r, _ = O.seam_wrap(g["r"], g["r_old"], np.radians(g["n"]), unit_deg=0)
+    assert np.array_equal(r_d, r_r)
+    assert np.allclose(np.radians(h_d), h_r, atol=1e-12)
+
+
+def test_seam_wrap_gluing_invariants(O):
+    """Crossing a border maps a point near it to a point near the glued border, rotates the
+    heading by the gluing rotation, and leaves inside points alone (idempotent)."""
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        t = rng.uniform(0.05, 0.95)
+        eps = rng.uniform(1e-4, 0.02)
+        cases = [  # (old, new, expected image, heading change in degrees)
+            ((0.3, t), (-eps, t), (t, eps), -90),            # left -> down edge, (x,y) -> (y,-x)
+            ((0.7, t), (1 + eps, t), (t, 1 - eps), -90),     # right -> up edge, (x,y) -> (y,2-x)
+            ((t, 0.7), (t, 1 + eps), (1 - eps, t), -270),    # up -> right edge, (x,y) -> (2-y,x)
+            ((t, 0.3), (t, -eps), (eps, t), -270),           # down -> left edge, (x,y) -> (-y,x)
+        ]
+        for old, new, exp, dh in cases:
+            st, r, _, h, _ = O.seam_wrap([new], [old], [0.0], unit_deg=1)
+            assert st == O.OK
+            assert np.allclose(r[0], exp, atol=1e-12), (old, new, r[0], exp)
+            assert h[0] == dh
+        st, r, _, h, cr = O.seam_wrap([[t, 0.5]], [[0.5, 0.5]], [10.0], unit_deg=1)
+        assert np.array_equal(r[0], [t, 0.5]) and h[0] == 10.0 and cr[0] == 0
+
+
+def test_seam_wrap_crossing_bound(O):
+    g = GOLD["seam_wrap"]
+    st, *_ = O.seam_wrap(g["r"], g["r_old"], g["n"], unit_deg=1, max_crossings=1)
+    assert st == O.E_TILING  # particle 0 needs two crossings
+
+
+def test_repulsive_adhesion_motion(O):
+    g = GOLD["repulsive_adhesion_motion"]
+    out = np.zeros(2)
+    for c in g["cases"]:
+        O.lib().or_repulsive_adhesion_motion(g["k"], g["sigma"], c["dist"], g["r_adh"], g["k_adh"], *c["dist_v"],
+                                             O.dp(out))
+        assert np.abs(out - np.array(c["expected"])).max() <= g["tol"]
+
+
+def test_repulsion_closed_form(O):
+    """|F| = k (2 sigma - d) / (2 sigma) |dv| / d below 2 sigma, pointing along -dv; adhesion at
+    2 sigma <= d <= r_adh (unreachable from the force loop, which skips d >= 2 sigma)."""
+    out = np.zeros(2)
+    k, s = 3.0, 0.4
+    for d in (0.05, 0.3, 0.79):
+        O.lib().or_repulsive_adhesion_motion(k, s, d, 1.0, 0.75, 0.3, -0.4, O.dp(out))
+        f = -k * (2 * s - d) / (2 * s)
+        assert np.allclose(out, [f * 0.3 / d, f * -0.4 / d], rtol=1e-14)
+    O.lib().or_repulsive_adhesion_motion(k, s, 0.9, 1.0, 0.75, 0.3, -0.4, O.dp(out))
+    f = 0.75 * (2 * s - 0.9) / (2 * s - 1.0)
+    assert np.allclose(out, [f * 0.3 / 0.9, f * -0.4 / 0.9], rtol=1e-14)
+
+
+def test_mean_unit_circle_vector_angle(O):
+    g = GOLD["mean_unit_circle_vector_angle_degrees"]
+    out = C.c_double()
+    assert O.lib().or_mean_unit_circle_vector_angle_degrees(None, 0, C.byref(out)) == O.E_INVALID
+    for c in g["cases"]:
+        a = np.array(c["angles"], np.float64)
+        assert O.lib().or_mean_unit_circle_vector_angle_degrees(O.dp(a), len(a), C.byref(out)) == O.OK
+        assert abs(out.value - c["expected"]) <= g["tol"]
+
+
+def test_transform_into_symmetric_matrix(O):
+    for c in GOLD["transform_into_symmetric_matrix"]["cases"]:
+        a = np.array(c["in"], np.float64)
+        O.lib().or_transform_into_symmetric_matrix(O.dp(a), 3)
+        assert np.array_equal(a, np.array(c["expected"], np.float64))
+
+
+def test_get_dist_vect(O):
+    g = GOLD["get_dist_vect"]
+    cases = g["cases"] + [dict(r=g["ten"]["r"], dx=g["ten"]["dx"], dy=g["ten"]["dy"])]
+    for c in cases:
+        r = np.array(c["r"], np.float64)
+        n = len(r)
+        dx = np.zeros((n, n)); dy = np.zeros((n, n))
+        O.lib().or_get_dist_vect(O.dp(r), n, O.dp(dx), O.dp(dy))
+        ex, ey = np.array(c["dx"]), np.array(c["dy"])
+        # Eigen isApprox(.., 1e-5): ||a - b|| <= 1e-5 min(||a||, ||b||); the printed vectors carry 6 digits
+        for a, b in ((dx, ex), (dy, ey)):
+            assert np.linalg.norm(a - b) <= 1e-5 * max(np.linalg.norm(b), 1.0) or np.abs(a - b).max() <= 1e-6
+
+
+def test_average_n_within_distance_reference_vector(O):
+    g = GOLD["average_n_within_distance"]
+    d = np.array(g["dist_length"], np.float64)
+    n = np.array(g["n"], np.int32)
+    assert O.lib().or_average_n_within_distance_deg(O.dp(d), 10, O.ip(n), g["sigma"]) == O.OK
+    # all rows, including the antipodal ones: the FP64 oracle reproduces the reference's 21
+    assert np.abs(n - np.array(g["expected"])).max() <= g["tol"]
+
+
+def test_count_particle_neighbors_thresholds(O):
+    """count = #{j : d != 0 and d <= 2.4 sigma}: boundary value included, zero excluded."""
+    s = 0.5
+    d = np.array([[0, 1.2, 1.2000001, 0], [1.2, 0, 0.3, 2], [1.2000001, 0.3, 0, 0], [0, 2, 0, 0]], np.float64)
+    out = np.zeros(4, np.int32)
+    O.lib().or_count_particle_neighbors(O.dp(d), 4, s, O.ip(out))
+    assert list(out) == [1, 2, 1, 0]
+
+
+def test_point_triangle_distance_brute_force(O):
+    """Closed-form region distance vs dense sampling of the triangle (Ericson's regions)."""
+    rng = np.random.default_rng(7)
+    f = O.lib().or_point_triangle_distance
+    f.argtypes = [O._D, O._D, O._D, O._D]
+    for _ in range(60):
+        a, b, c = (rng.uniform(-1, 1, 2) for _ in range(3))
+        if abs((b - a)[0] * (c - a)[1] - (b - a)[1] * (c - a)[0]) < 0.05:
+            continue
+        # barycentric lattice: every point of the triangle is within h of a sample
+        k = 200
+        i, j = np.meshgrid(np.arange(k + 1), np.arange(k + 1))
+        keep = i + j <= k
+        u, v = i[keep] / k, j[keep] / k
+        pts = a + u[:, None] * (b - a) + v[:, None] * (c - a)
+        h = max(np.linalg.norm(b - a), np.linalg.norm(c - a), np.linalg.norm(c - b)) / k
+        for _ in range(10):
+            p = rng.uniform(-2, 2, 2)
+            got = f(O.dp(np.ascontiguousarray(p)), O.dp(a.copy()), O.dp(b.copy()), O.dp(c.copy()))
+            brute = np.sqrt(((pts - p) ** 2).sum(1)).min()
+            assert got <= brute + 1e-12 and brute - got <= h
+    # inside points are at distance ~0
+    p = np.array([0.2, 0.2]); a = np.array([0.0, 0.0]); b = np.array([1.0, 0.0]); c = np.array([0.0, 1.0])
+    assert f(O.dp(p), O.dp(a), O.dp(b), O.dp(c)) < 1e-15
+
+
+def test_order_parameter_closed_form(O):
+    """z := 0 reading: phi = |sum_i sign(x_i vy_i - y_i vx_i)| / N."""
+    rng = np.random.default_rng(3)
+    r = rng.uniform(0, 1, (50, 2)); v = rng.normal(size=(50, 2))
+    s = np.sign(r[:, 0] * v[:, 1] - r[:, 1] * v[:, 0])
+    got = O.lib().or_order_parameter(O.dp(r), O.dp(v), 50)
+    assert got == abs(s.sum()) / 50
+    v2 = np.stack([-r[:, 1], r[:, 0]], 1)  # all counter-clockwise
+    assert O.lib().or_order_parameter(O.dp(r), O.dp(np.ascontiguousarray(v2)), 50) == 1.0
+
+
+@pytest.fixture(scope="module")
+def mesh1(T, O):
+    import parity as P
+    m, tb = P.mesh_and_tables(8, 0.2)
+    return m, tb, O.Mesh(tb)
+
+
+def test_remap_bucketed_equals_brute_force(O, mesh1):
+    _, tb, om = mesh1
+    rng = np.random.default_rng(11)
+    x = rng.random(3000); y = rng.random(3000)
+    # plus points exactly on UV vertices and edge midpoints, and the square's corners
+    uv = tb["uv"]; f = tb["faces"]
+    x = np.concatenate([x, uv[:, 0], 0.5 * (uv[f[:, 0], 0] + uv[f[:, 1], 0]), [0, 1, 0, 1]])
+    y = np.concatenate([y, uv[:, 1], 0.5 * (uv[f[:, 0], 1] + uv[f[:, 1], 1]), [0, 0, 1, 1]])
+    a = O.remap(om, x, y, 0); b = O.remap(om, x, y, 1)
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2]) and np.array_equal(a[0], b[0])
+
+
+def test_remap_closed_form(O, mesh1):
+    """Face = the UV triangle containing the point; r3d = distance-weighted blend of its 3-D
+    corners (weights = normalised distances, CellHelper.cpp:133-141); vertex = corner nearest r3d."""
+    _, tb, om = mesh1
+    uv, xyz, f = tb["uv"], tb["xyz"], tb["faces"]
+    rng = np.random.default_rng(2)
+    fi = rng.integers(0, len(f), 300)
+    u = rng.uniform(0.05, 0.9, 300); v = rng.uniform(0.05, 0.9, 300); m = u + v > 0.95
+    u[m] *= 0.5; v[m] *= 0.5
+    A, B, Cc = uv[f[fi, 0]], uv[f[fi, 1]], uv[f[fi, 2]]
+    p = A + u[:, None] * (B - A) + v[:, None] * (Cc - A)
+    r3d, vid, face = O.remap(om, p[:, 0], p[:, 1], 0)
+    assert np.array_equal(face, fi)
+    w = np.stack([np.linalg.norm(p - A, axis=1), np.linalg.norm(p - B, axis=1), np.linalg.norm(p - Cc, axis=1)], 1)
+    w /= w.sum(1, keepdims=True)
+    X = w[:, :1] * xyz[f[fi, 0]] + w[:, 1:2] * xyz[f[fi, 1]] + w[:, 2:] * xyz[f[fi, 2]]
+    assert np.abs(X - r3d).max() < 1e-12
+    dist = np.stack([np.linalg.norm(X - xyz[f[fi, q]], axis=1) for q in range(3)], 1)
+    assert np.array_equal(vid, f[fi, dist.argmin(1)])
+
+
+def test_step_isolated_particle(O, mesh1):
+    """A lone particle: no force, mean heading = own heading, so it moves v0 dt along theta and
+    its heading changes by the noise only."""
+    _, tb, om = mesh1
+    p = dict(sigma=0.05, v0=0.3, k=10.0, mu=1.0, tau=0.5, dt=0.01)
+    x, y, th = [0.41], [0.37], [0.8]
+    _, vid, _ = O.remap(om, x, y, 0)
+    st, s, o = O.step(om, p, x, y, th, vid, [0.01], remap_mode=0)
+    assert st == O.OK
+    assert math.isclose(s["x"][0], 0.41 + 0.3 * math.cos(0.8) * 0.01, rel_tol=1e-14)
+    assert math.isclose(s["y"][0], 0.37 + 0.3 * math.sin(0.8) * 0.01, rel_tol=1e-14)
+    assert math.isclose(s["theta"][0], 0.81, rel_tol=1e-14)
+    assert o["nbr"][0] == 0
+
+
+def test_step_permutation_equivariance(O, mesh1):
+    _, tb, om = mesh1
+    rng = np.random.default_rng(4)
+    n = 60
+    x = rng.uniform(0.3, 0.5, n); y = rng.uniform(0.3, 0.5, n); th = rng.uniform(-3, 3, n); eta = rng.normal(0, .1, n)
+    _, vid, _ = O.remap(om, x, y, 0)
+    p = dict(sigma=0.1, v0=0.2, k=1.0, mu=1.0, tau=0.1, dt=0.01)
+    st, s, o = O.step(om, p, x, y, th, vid, eta)
+    perm = rng.permutation(n)
+    st2, s2, o2 = O.step(om, p, x[perm], y[perm], th[perm], vid[perm], eta[perm])
+    assert st == st2 == O.OK
+    for k in ("x", "y", "theta"):
+        assert np.allclose(s[k][perm], s2[k], rtol=1e-12, atol=1e-13)
+    assert np.array_equal(o["nbr"][perm], o2["nbr"])
+
+
+def test_step_aligned_flock_stays_aligned(O, mesh1):
+    _, tb, om = mesh1
+    rng = np.random.default_rng(6)
+    n = 40
+    x = rng.uniform(0.4, 0.6, n); y = rng.uniform(0.4, 0.6, n); th = np.full(n, 0.7)
+    _, vid, _ = O.remap(om, x, y, 0)
+    p = dict(sigma=0.1, v0=0.2, k=1.0, mu=1.0, tau=0.05, dt=0.01)
+    st, s, o = O.step(om, p, x, y, th, vid, None)
+    assert st == O.OK
+    assert np.abs(s["theta"] - 0.7).max() < 1e-12
+    assert abs(o["phi_polar"] - 1.0) < 1e-12
+
+
+def test_step_tau_equals_dt_reduces_to_mean(O, mesh1):
+    """tau = dt, eta = 0: the continuous rule returns the reference's circular mean (in radians)."""
+    _, tb, om = mesh1
+    rng = np.random.default_rng(8)
+    n = 80
+    x = rng.uniform(0.3, 0.5, n); y = rng.uniform(0.3, 0.5, n); th = rng.uniform(-3, 3, n)
+    _, vid, _ = O.remap(om, x, y, 0)
+    p = dict(sigma=0.1, v0=1e-6, k=0.0, mu=1.0, tau=0.01, dt=0.01)
+    st, s, _ = O.step(om, p, x, y, th, vid, None)
+    sets = O.neighbor_sets(om, vid, p["sigma"])
+    for i in range(n):
+        a = np.degrees(th[sets[i]])
+        out = C.c_double()
+        O.lib().or_mean_unit_circle_vector_angle_degrees(O.dp(np.ascontiguousarray(a)), len(a), C.byref(out))
+        d = (np.radians(out.value) - s["theta"][i] + np.pi) % (2 * np.pi) - np.pi
+        assert abs(d) < 1e-12
+
+
+def test_step_int_degrees_runs(O, mesh1):
+    _, tb, om = mesh1
+    rng = np.random.default_rng(9)
+    n = 50
+    x = rng.uniform(0.2, 0.8, n); y = rng.uniform(0.2, 0.8, n); nd = rng.integers(0, 360, n)
+    _, vid, _ = O.remap(om, x, y, 0)
+    p = dict(sigma=0.1, v0=0.1, k=1.0, dt=0.001)
+    st, s, o = O.step_int_degrees(om, p, x, y, nd, vid)
+    assert st == O.OK
+    assert s["n"].dtype == np.int32 and (s["n"] >= -360 * 4).all()
+    assert ((s["x"] >= 0) & (s["x"] <= 1) & (s["y"] >= 0) & (s["y"] <= 1)).all()
+
+
+def test_step_nonfinite_reported(O, mesh1):
+    _, tb, om = mesh1
+    p = dict(sigma=0.1, v0=0.1, k=1.0, mu=1.0, tau=1.0, dt=0.01)
+    st, *_ = O.step(om, p, [0.5, 0.52], [0.5, 0.5], [0.0, 1.0], O.remap(om, [0.5, 0.52], [0.5, 0.5], 0)[1],
+                    [float("nan"), 0.0])
+    assert st == O.E_NONFINITE
