@@ -85,6 +85,22 @@ SIGNATURES = [
     ("tissue_set_launch", C.c_int32, [_P, C.c_int32, C.c_int32]),
     ("tissue_synchronize", C.c_int32, [_P]),
     ("tissue_last_step_ms", C.c_int32, [_P, _D]),
+    ("tissue_set_mode", C.c_int32, [_P, C.c_int32]),
+    ("tissue_resident_capacity", C.c_int32, [_P, _I]),
+    ("tissue_batch_create", C.c_int32, [_P, C.c_int32, _I, C.POINTER(Params), C.POINTER(_P)]),
+    ("tissue_batch_destroy", None, [_P]),
+    ("tissue_batch_init_random", C.c_int32, [_P, C.POINTER(C.c_uint64)]),
+    ("tissue_batch_set_state", C.c_int32, [_P, _F, _F, _F, _I]),
+    ("tissue_batch_get_state", C.c_int32, [_P, _F, _F, _F, _I]),
+    ("tissue_batch_get_outputs", C.c_int32, [_P, _F, _F, _F, _I, _I]),
+    ("tissue_batch_stage_noise", C.c_int32, [_P, _F, C.c_int32]),
+    ("tissue_batch_gen_noise", C.c_int32, [_P, C.POINTER(C.c_uint64), _D, C.c_int32]),
+    ("tissue_batch_run", C.c_int32, [_P, C.c_int32, C.c_int32]),
+    ("tissue_batch_get_order", C.c_int32, [_P, C.c_int32, _D, _D, C.c_int32, _I]),
+    ("tissue_batch_get_flags", C.c_int32, [_P, C.POINTER(C.c_uint32), _I]),
+    ("tissue_batch_set_launch", C.c_int32, [_P, C.c_int32, C.c_int32, C.c_int32]),
+    ("tissue_batch_autotune", C.c_int32, [_P, C.c_int32, _I, _I]),
+    ("tissue_batch_last_run_ms", C.c_int32, [_P, _D]),
     ("tissue_gen_noise", C.c_int32, [C.c_uint64, C.c_double, C.c_int32, C.c_int32, _F]),
     ("tissue_philox4x32", None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
 ]
@@ -290,6 +306,9 @@ class Sim:
     def set_launch(self, lanes: int = 0, block: int = 0) -> None:
         _check(lib().tissue_set_launch(self.h, lanes, block))
 
+    def set_mode(self, mode: int) -> None:
+        _check(lib().tissue_set_mode(self.h, mode))
+
     def synchronize(self) -> None:
         _check(lib().tissue_synchronize(self.h))
 
@@ -301,6 +320,109 @@ class Sim:
     def close(self):
         if getattr(self, "h", None):
             lib().tissue_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+MODE_AUTO, MODE_STEP_KERNEL, MODE_RESIDENT = 0, 1, 2
+
+
+def resident_capacity(mesh: Mesh) -> int:
+    c = C.c_int32()
+    _check(lib().tissue_resident_capacity(mesh.h, C.byref(c)))
+    return c.value
+
+
+class Batch:
+    """Independent replicas run by the persistent kernel (tissue_batch)."""
+
+    def __init__(self, mesh: Mesh, sizes, params_list):
+        self.mesh = mesh
+        self.sizes = np.ascontiguousarray(sizes, np.int32)
+        self.R = len(self.sizes)
+        self.off = np.concatenate([[0], np.cumsum(self.sizes)[:-1]]).astype(np.int64)
+        self.ntot = int(self.sizes.sum())
+        ps = (Params * self.R)(*params_list)
+        h = C.c_void_p()
+        _check(lib().tissue_batch_create(mesh.h, self.R, _ptr(self.sizes, C.c_int32), ps, C.byref(h)))
+        self.h = h
+
+    def init_random(self, seeds) -> None:
+        s = np.ascontiguousarray(seeds, np.uint64)
+        _check(lib().tissue_batch_init_random(self.h, s.ctypes.data_as(C.POINTER(C.c_uint64))))
+
+    def set_state(self, x, y, theta, vid=None) -> None:
+        x = np.ascontiguousarray(x, np.float32); y = np.ascontiguousarray(y, np.float32)
+        t = np.ascontiguousarray(theta, np.float32)
+        v = None if vid is None else np.ascontiguousarray(vid, np.int32)
+        _check(lib().tissue_batch_set_state(self.h, _ptr(x, C.c_float), _ptr(y, C.c_float), _ptr(t, C.c_float),
+                                            _ptr(v, C.c_int32)))
+
+    def get_state(self) -> State:
+        n = self.ntot
+        s = State(np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros(n, np.int32))
+        _check(lib().tissue_batch_get_state(self.h, _ptr(s.x, C.c_float), _ptr(s.y, C.c_float),
+                                            _ptr(s.theta, C.c_float), _ptr(s.vid, C.c_int32)))
+        return s
+
+    def get_outputs(self) -> Outputs:
+        n = self.ntot
+        o = Outputs(np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros((n, 3), np.float32),
+                    np.zeros(n, np.int32), np.zeros(n, np.int32))
+        _check(lib().tissue_batch_get_outputs(self.h, _ptr(o.vx, C.c_float), _ptr(o.vy, C.c_float),
+                                              _ptr(o.r3d, C.c_float), _ptr(o.face, C.c_int32),
+                                              _ptr(o.nbr, C.c_int32)))
+        return o
+
+    def stage_noise(self, noise: np.ndarray) -> None:
+        noise = np.ascontiguousarray(noise, np.float32).reshape(-1, self.ntot)
+        _check(lib().tissue_batch_stage_noise(self.h, _ptr(noise, C.c_float), noise.shape[0]))
+
+    def gen_noise(self, seeds, stds, n_rows: int) -> None:
+        s = np.ascontiguousarray(seeds, np.uint64); d = np.ascontiguousarray(stds, np.float64)
+        _check(lib().tissue_batch_gen_noise(self.h, s.ctypes.data_as(C.POINTER(C.c_uint64)), _ptr(d, C.c_double),
+                                            n_rows))
+
+    def run(self, n_steps: int, first_row: int = -1) -> int:
+        code = lib().tissue_batch_run(self.h, n_steps, first_row)
+        if code not in (OK, E_LOST_PARTICLE, E_NONFINITE, E_TILING_NO_CONVERGE):
+            _check(code)
+        return code
+
+    def order(self, r: int) -> tuple[np.ndarray, np.ndarray]:
+        cnt = C.c_int32()
+        _check(lib().tissue_batch_get_order(self.h, r, None, None, 0, C.byref(cnt)))
+        a = np.zeros(cnt.value); b = np.zeros(cnt.value)
+        _check(lib().tissue_batch_get_order(self.h, r, _ptr(a, C.c_double), _ptr(b, C.c_double), cnt.value,
+                                            C.byref(cnt)))
+        return a, b
+
+    def flags(self) -> tuple[np.ndarray, np.ndarray]:
+        f = np.zeros(self.R, np.uint32); s = np.zeros(self.R, np.int32)
+        _check(lib().tissue_batch_get_flags(self.h, f.ctypes.data_as(C.POINTER(C.c_uint32)), _ptr(s, C.c_int32)))
+        return f, s
+
+    def set_launch(self, lanes: int = 0, block: int = 0, steps_per_launch: int = 0) -> None:
+        _check(lib().tissue_batch_set_launch(self.h, lanes, block, steps_per_launch))
+
+    def autotune(self, n_steps: int) -> tuple[int, int]:
+        l = C.c_int32(); b = C.c_int32()
+        _check(lib().tissue_batch_autotune(self.h, n_steps, C.byref(l), C.byref(b)))
+        return l.value, b.value
+
+    def last_run_ms(self) -> float:
+        ms = C.c_double()
+        _check(lib().tissue_batch_last_run_ms(self.h, C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().tissue_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

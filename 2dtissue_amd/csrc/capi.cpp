@@ -11,140 +11,49 @@
 #include <string>
 #include <vector>
 
-#include "../../include/tissue.h"
-#include "launch.h"
-#include "mesh.h"
+#include "capi_common.h"
 #include "rng.h"
 
 using tissue::OrderPartial;
 using tissue::StepIO;
+using namespace tcapi;
+
+thread_local std::string tcapi::g_err;
 
 namespace {
 
-thread_local std::string g_err;
-
-struct ApiError : std::runtime_error {
-  int32_t code;
-  ApiError(int32_t c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-#define HIP_OK(expr)                                                                                  \
-  do {                                                                                                \
-    hipError_t e_ = (expr);                                                                           \
-    if (e_ != hipSuccess) throw ApiError(TISSUE_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
-  } while (0)
-
-template <class F>
-int32_t guarded(F&& f) {
-  try {
-    return f();
-  } catch (const ApiError& e) {
-    g_err = e.what();
-    return e.code;
-  } catch (const std::bad_alloc&) {
-    g_err = "host out of memory";
-    return TISSUE_E_INVALID_ARG;
-  } catch (const std::exception& e) {
-    g_err = e.what();
-    return TISSUE_E_MESH;
-  }
-}
-
-template <class T>
-T* dalloc(size_t n) {
-  void* p = nullptr;
-  HIP_OK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
-  return (T*)p;
-}
-
-template <class T>
-void dfree(T*& p) {
-  if (p) (void)hipFree((void*)p);
-  p = nullptr;
-}
-
-template <class T>
-void upload(T* dst, const T* src, size_t n) {
-  if (n) HIP_OK(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyHostToDevice));
-}
-
-// largest float f with (double)f < T (strict) or <= T: FP32 thresholds that reproduce the
-// FP64 comparisons of the reference exactly for FP32 table distances
-float float_below(double T, bool strict) {
-  float f = (float)T;
-  auto bad = [&](float x) { return strict ? !((double)x < T) : !((double)x <= T); };
-  while (bad(f)) f = std::nextafter(f, -INFINITY);
-  for (;;) {
-    float g = std::nextafter(f, INFINITY);
-    if (bad(g)) break;
-    f = g;
-  }
-  return f;
-}
-
 }  // namespace
 
-// ------------------------------------------------------------------ mesh
-
-struct tissue_mesh {
-  tissue::UVMesh m;
-  bool tables = false;
-  double sigma_max = 0.0;
-  std::mutex mu;
-  bool dev_ready = false;
-  int device = -1;
-  float2* d_uv = nullptr;
-  float* d_xyz = nullptr;
-  int32_t* d_faces = nullptr;
-  int32_t* d_cell_start = nullptr;
-  int32_t* d_cell_faces = nullptr;
-  int32_t* d_row_ptr = nullptr;
-  int2* d_nbr = nullptr;
-
-  void free_device() {
-    if (!dev_ready) return;
-    int cur = 0;
-    (void)hipGetDevice(&cur);
-    (void)hipSetDevice(device);
-    dfree(d_uv); dfree(d_xyz); dfree(d_faces); dfree(d_cell_start); dfree(d_cell_faces); dfree(d_row_ptr); dfree(d_nbr);
-    (void)hipSetDevice(cur);
-    dev_ready = false;
+void tcapi::random_init(const tissue::UVMesh& M, uint64_t seed, int n, float* x, float* y, float* th) {
+  std::vector<double> cdf(M.nf);
+  double tot = 0;
+  for (int f = 0; f < M.nf; ++f) {
+    const double* A = &M.xyz[3 * M.faces[3 * f]];
+    const double* B = &M.xyz[3 * M.faces[3 * f + 1]];
+    const double* C = &M.xyz[3 * M.faces[3 * f + 2]];
+    double e1[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]}, e2[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
+    double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2], cz = e1[0] * e2[1] - e1[1] * e2[0];
+    tot += 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+    cdf[f] = tot;
   }
-
-  tdev::MeshDev device_view() {
-    std::lock_guard<std::mutex> lk(mu);
-    if (!tables) throw ApiError(TISSUE_E_INVALID_ARG, "mesh tables not built (tissue_mesh_build_tables)");
-    int cur = 0;
-    HIP_OK(hipGetDevice(&cur));
-    if (dev_ready && device != cur) throw ApiError(TISSUE_E_INVALID_ARG, "mesh already resident on another device");
-    if (!dev_ready) {
-      device = cur;
-      std::vector<float2> uv(m.nv);
-      std::vector<float> xyz(3 * (size_t)m.nv);
-      for (int v = 0; v < m.nv; ++v) uv[v] = make_float2((float)m.uv[2 * v], (float)m.uv[2 * v + 1]);
-      for (size_t q = 0; q < xyz.size(); ++q) xyz[q] = (float)m.xyz[q];
-      std::vector<int2> nbr(m.col.size());
-      for (size_t e = 0; e < nbr.size(); ++e) {
-        int bits;
-        std::memcpy(&bits, &m.dist[e], 4);
-        nbr[e] = make_int2(m.col[e], bits);
-      }
-      d_uv = dalloc<float2>(uv.size()); upload(d_uv, uv.data(), uv.size());
-      d_xyz = dalloc<float>(xyz.size()); upload(d_xyz, xyz.data(), xyz.size());
-      d_faces = dalloc<int32_t>(m.faces.size()); upload(d_faces, m.faces.data(), m.faces.size());
-      d_cell_start = dalloc<int32_t>(m.cell_start.size()); upload(d_cell_start, m.cell_start.data(), m.cell_start.size());
-      d_cell_faces = dalloc<int32_t>(m.cell_faces.size()); upload(d_cell_faces, m.cell_faces.data(), m.cell_faces.size());
-      d_row_ptr = dalloc<int32_t>(m.row_ptr.size()); upload(d_row_ptr, m.row_ptr.data(), m.row_ptr.size());
-      d_nbr = dalloc<int2>(nbr.size()); upload(d_nbr, nbr.data(), nbr.size());
-      dev_ready = true;
-    }
-    tdev::MeshDev v;
-    v.nv = m.nv; v.nf = m.nf; v.grid = m.grid; v.pad = 0;
-    v.uv = d_uv; v.xyz = d_xyz; v.faces = d_faces; v.cell_start = d_cell_start; v.cell_faces = d_cell_faces;
-    v.row_ptr = d_row_ptr; v.nbr = d_nbr;
-    return v;
+  for (int i = 0; i < n; ++i) {
+    const tissue::Draw4 d = tissue::philox_draw(seed, (uint32_t)i, 0u, 0u);
+    const double u0 = tissue::u53(d.w[0], d.w[1]);
+    int f = (int)(std::upper_bound(cdf.begin(), cdf.end(), u0 * tot) - cdf.begin());
+    f = std::min(f, M.nf - 1);
+    const tissue::Draw4 e = tissue::philox_draw(seed, (uint32_t)i, 0u, 2u);
+    double r1 = tissue::u53(d.w[2], d.w[3]), r2 = tissue::u53(e.w[0], e.w[1]);
+    if (r1 + r2 > 1.0) { r1 = 1.0 - r1; r2 = 1.0 - r2; }
+    const double* a = &M.uv[2 * M.faces[3 * f]];
+    const double* b = &M.uv[2 * M.faces[3 * f + 1]];
+    const double* c = &M.uv[2 * M.faces[3 * f + 2]];
+    x[i] = (float)std::clamp(a[0] + r1 * (b[0] - a[0]) + r2 * (c[0] - a[0]), 0.0, 1.0);
+    y[i] = (float)std::clamp(a[1] + r1 * (b[1] - a[1]) + r2 * (c[1] - a[1]), 0.0, 1.0);
+    double t = 2.0 * M_PI * tissue::u53(e.w[2], e.w[3]);
+    t = t - 2.0 * M_PI * std::floor((t + M_PI) / (2.0 * M_PI));
+    th[i] = (float)t;
   }
-};
+}
 
 namespace {
 
@@ -291,8 +200,19 @@ struct tissue_sim {
   int sphase = 0;  // state buffer holding the current state
   int bphase = 0;  // bucket set holding the current state's buckets
   int lanes = 0, block = 0;
+  int mode = TISSUE_MODE_AUTO;
+  int resident_cap = 0;  // largest n the persistent kernel holds for this mesh
+  tissue::ReplicaDesc* d_rep = nullptr;
+  bool buckets_stale = false;
   bool has_state = false;
   double last_ms = 0.0;
+
+  bool use_resident() const {
+    if (mode == TISSUE_MODE_STEP_KERNEL) return false;
+    return n <= resident_cap;
+  }
+  int res_lanes() const { return lanes > 0 ? std::min(lanes, 8) : tissue::resident_default_lanes(n); }
+  int res_block() const { return block > 0 ? std::min(block, tissue::kResidentMaxBlock) : tissue::kResidentMaxBlock; }
 
   tdev::BucketsIn bin(int k) const { return {bcount[k], bslot[k], bovf_n[k], bovf[k]}; }
   tdev::BucketsOut bout(int k) const { return {bcount[k], bslot[k], bovf_n[k], bovf[k]}; }
@@ -302,7 +222,7 @@ struct tissue_sim {
     for (int k = 0; k < 2; ++k) { dfree(xycs[k]); dfree(th[k]); dfree(vid[k]); }
     for (int k = 0; k < 3; ++k) { dfree(bcount[k]); dfree(bslot[k]); dfree(bovf_n[k]); dfree(bovf[k]); }
     dfree(vx); dfree(vy); dfree(r3x); dfree(r3y); dfree(r3z); dfree(face); dfree(nbr);
-    dfree(flags); dfree(first_bad); dfree(noise); dfree(partial); dfree(phi);
+    dfree(flags); dfree(first_bad); dfree(noise); dfree(partial); dfree(phi); dfree(d_rep);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -325,39 +245,6 @@ void check_sim(tissue_sim* s) {
   HIP_OK(hipSetDevice(s->device));
 }
 
-void check_params(const tissue_params* p, double sigma_max) {
-  if (!p) throw ApiError(TISSUE_E_INVALID_ARG, "null params");
-  const double v[] = {p->sigma, p->v0, p->k, p->mu, p->tau, p->dt};
-  for (double x : v)
-    if (!std::isfinite(x)) throw ApiError(TISSUE_E_INVALID_ARG, "non-finite parameter");
-  if (!(p->sigma > 0) || !(p->tau > 0) || !(p->dt > 0)) throw ApiError(TISSUE_E_INVALID_ARG, "sigma, tau and dt must be positive");
-  if (p->sigma > sigma_max * (1 + 1e-12))
-    throw ApiError(TISSUE_E_INVALID_ARG, "sigma exceeds the sigma_max the mesh tables were built for");
-}
-
-// the reference's tiling loop has no bound; this one only stops pathological inputs
-constexpr int kDefaultMaxCrossings = 16384;
-
-tdev::StepParams make_step_params(const tissue_params& p) {
-  tdev::StepParams s{};
-  s.t_near = float_below(2 * p.sigma, true);
-  s.t_count = float_below(2.4 * p.sigma, false);
-  s.two_sigma = (float)(2 * p.sigma);
-  s.k = (float)p.k;
-  s.v0 = (float)p.v0;
-  s.mu = (float)p.mu;
-  s.dt = (float)p.dt;
-  s.rate = (float)(p.dt / p.tau);
-  s.max_crossings = p.max_crossings > 0 ? p.max_crossings : kDefaultMaxCrossings;
-  s.two_sigma64 = 2 * p.sigma;
-  s.k64 = p.k;
-  s.v064 = p.v0;
-  s.mu64 = p.mu;
-  s.dt64 = p.dt;
-  s.rate64 = p.dt / p.tau;
-  return s;
-}
-
 void rebuild_buckets(tissue_sim* s) {
   for (int k = 0; k < 3; ++k) HIP_OK(tissue::launch_clear(s->md.nv, s->bcount[k], s->bovf_n[k], s->stream));
   HIP_OK(tissue::launch_insert(s->n, s->vid[s->sphase], s->bout(s->bphase), s->stream));
@@ -370,18 +257,43 @@ int32_t status_from_flags(unsigned f) {
   return TISSUE_OK;
 }
 
+// steps per launch of the persistent kernel (bounds one kernel's duration)
+constexpr int kResidentStepsPerLaunch = 256;
+
+// collect the order parameters and flags of a run_steps call; first_bad is relative to the call
+// for the resident kernel and absolute for the per-step kernel
+int32_t finish_steps(tissue_sim* s, int n_steps, bool relative_first_bad) {
+  std::vector<double> phi(2 * (size_t)n_steps);
+  unsigned fl = 0;
+  int fb = INT_MAX;
+  HIP_OK(hipMemcpyAsync(phi.data(), s->phi, phi.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipMemcpyAsync(&fl, s->flags, sizeof fl, hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipMemcpyAsync(&fb, s->first_bad, sizeof fb, hipMemcpyDeviceToHost, s->stream));
+  HIP_OK(hipStreamSynchronize(s->stream));
+  float ms = 0.f;
+  HIP_OK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+  s->last_ms = ms;
+  for (int t = 0; t < n_steps; ++t) {
+    s->hist_ref.push_back(phi[2 * t]);
+    s->hist_polar.push_back(phi[2 * t + 1]);
+  }
+  if (fl && relative_first_bad) fb = (int)std::min<long>(s->steps_done + fb, INT_MAX - 1);
+  s->steps_done += n_steps;
+  s->flags_acc |= fl;
+  if (fl && fb < s->first_bad_acc) s->first_bad_acc = fb;
+  const int32_t st = status_from_flags(fl);
+  if (st != TISSUE_OK) {
+    char buf[128];
+    std::snprintf(buf, sizeof buf, "step %d raised validation flags 0x%x", fb, fl);
+    g_err = buf;
+  }
+  return st;
+}
+
 int32_t run_steps(tissue_sim* s, int n_steps, const float* noise_dev, size_t noise_stride) {
   if (!s->has_state) throw ApiError(TISSUE_E_INVALID_ARG, "no state: call tissue_init_random or tissue_set_state");
   if (n_steps < 0) throw ApiError(TISSUE_E_INVALID_ARG, "negative step count");
   if (n_steps == 0) return TISSUE_OK;
-  const int lanes = s->eff_lanes(), block = s->eff_block();
-  const int blocks = tissue::step_blocks(s->n, lanes, block);
-  const size_t need_partial = (size_t)n_steps * blocks;
-  if (s->partial_cap < need_partial) {
-    dfree(s->partial);
-    s->partial = dalloc<OrderPartial>(need_partial);
-    s->partial_cap = need_partial;
-  }
   if (s->phi_cap < 2 * (size_t)n_steps) {
     dfree(s->phi);
     s->phi = dalloc<double>(2 * (size_t)n_steps);
@@ -391,6 +303,43 @@ int32_t run_steps(tissue_sim* s, int n_steps, const float* noise_dev, size_t noi
   const int big = INT_MAX;
   HIP_OK(hipMemcpyAsync(s->flags, &zero, sizeof zero, hipMemcpyHostToDevice, s->stream));
   HIP_OK(hipMemcpyAsync(s->first_bad, &big, sizeof big, hipMemcpyHostToDevice, s->stream));
+  if (s->use_resident()) {
+    // persistent LDS-resident kernel, in place on the current state buffers
+    const int lanes = s->res_lanes(), block = s->res_block();
+    tissue::ResidentIO io;
+    const int cur = s->sphase;
+    io.xycs = s->xycs[cur]; io.th = s->th[cur]; io.vid = s->vid[cur];
+    io.stage_xycs = s->xycs[1 - cur]; io.stage_th = s->th[1 - cur]; io.stage_vid = s->vid[1 - cur];
+    io.vx = s->vx; io.vy = s->vy; io.r3x = s->r3x; io.r3y = s->r3y; io.r3z = s->r3z;
+    io.face = s->face; io.nbr = s->nbr;
+    io.noise_stride = noise_stride;
+    io.phi = s->phi;
+    io.phi_stride = n_steps;
+    io.flags = s->flags;
+    io.first_bad = s->first_bad;
+    HIP_OK(hipEventRecord(s->ev0, s->stream));
+    for (int t0 = 0; t0 < n_steps; t0 += kResidentStepsPerLaunch) {
+      const int k = std::min(kResidentStepsPerLaunch, n_steps - t0);
+      io.noise = noise_dev ? noise_dev + (size_t)t0 * noise_stride : nullptr;
+      io.t_base = t0;
+      HIP_OK(tissue::launch_resident(lanes, block, s->md, s->d_rep, 1, s->n, k, io, s->stream));
+    }
+    HIP_OK(hipEventRecord(s->ev1, s->stream));
+    s->buckets_stale = true;
+    return finish_steps(s, n_steps, true);
+  }
+  if (s->buckets_stale) {
+    rebuild_buckets(s);
+    s->buckets_stale = false;
+  }
+  const int lanes = s->eff_lanes(), block = s->eff_block();
+  const int blocks = tissue::step_blocks(s->n, lanes, block);
+  const size_t need_partial = (size_t)n_steps * blocks;
+  if (s->partial_cap < need_partial) {
+    dfree(s->partial);
+    s->partial = dalloc<OrderPartial>(need_partial);
+    s->partial_cap = need_partial;
+  }
   HIP_OK(hipEventRecord(s->ev0, s->stream));
   for (int t = 0; t < n_steps; ++t) {
     const int si = s->sphase, so = 1 - s->sphase;
@@ -412,30 +361,7 @@ int32_t run_steps(tissue_sim* s, int n_steps, const float* noise_dev, size_t noi
   }
   HIP_OK(hipEventRecord(s->ev1, s->stream));
   HIP_OK(tissue::launch_order_finalize(s->partial, n_steps, blocks, s->n, s->phi, s->stream));
-  std::vector<double> phi(2 * (size_t)n_steps);
-  unsigned fl = 0;
-  int fb = INT_MAX;
-  HIP_OK(hipMemcpyAsync(phi.data(), s->phi, phi.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
-  HIP_OK(hipMemcpyAsync(&fl, s->flags, sizeof fl, hipMemcpyDeviceToHost, s->stream));
-  HIP_OK(hipMemcpyAsync(&fb, s->first_bad, sizeof fb, hipMemcpyDeviceToHost, s->stream));
-  HIP_OK(hipStreamSynchronize(s->stream));
-  float ms = 0.f;
-  HIP_OK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
-  s->last_ms = ms;
-  for (int t = 0; t < n_steps; ++t) {
-    s->hist_ref.push_back(phi[2 * t]);
-    s->hist_polar.push_back(phi[2 * t + 1]);
-  }
-  s->steps_done += n_steps;
-  s->flags_acc |= fl;
-  if (fl && fb < s->first_bad_acc) s->first_bad_acc = fb;
-  const int32_t st = status_from_flags(fl);
-  if (st != TISSUE_OK) {
-    char buf[128];
-    std::snprintf(buf, sizeof buf, "step %d raised validation flags 0x%x", fb, fl);
-    g_err = buf;
-  }
-  return st;
+  return finish_steps(s, n_steps, false);
 }
 
 }  // namespace
@@ -476,6 +402,11 @@ int32_t tissue_create(const tissue_mesh* mc, const tissue_params* p, int32_t n, 
       s->face = dalloc<int>(n); s->nbr = dalloc<int>(n);
       s->flags = dalloc<unsigned>(1);
       s->first_bad = dalloc<int>(1);
+      s->resident_cap = tissue::resident_max_particles(s->md.nv);
+      if (n <= s->resident_cap) HIP_OK(tissue::resident_prepare(s->md.nv, n));
+      tissue::ReplicaDesc rep{0, n, s->sp};
+      s->d_rep = dalloc<tissue::ReplicaDesc>(1);
+      upload(s->d_rep, &rep, 1);
     } catch (...) {
       s->release();
       delete s;
@@ -517,6 +448,7 @@ int32_t tissue_set_state(tissue_sim* s, const float* x, const float* y, const fl
       HIP_OK(hipMemsetAsync(s->vy, 0, n * sizeof(float), s->stream));
       HIP_OK(hipMemsetAsync(s->nbr, 0, n * sizeof(int), s->stream));
       rebuild_buckets(s);
+      s->buckets_stale = false;
       HIP_OK(hipStreamSynchronize(s->stream));
     } catch (...) {
       dfree(dx); dfree(dy); dfree(dt);
@@ -531,39 +463,9 @@ int32_t tissue_set_state(tissue_sim* s, const float* x, const float* y, const fl
 int32_t tissue_init_random(tissue_sim* s, uint64_t seed) {
   return guarded([&] {
     check_sim(s);
-    const auto& M = s->mesh->m;
-    // face chosen with probability proportional to its 3-D area, then a uniform point of the
-    // triangle (its UV image is affine, so uniform in UV too); heading uniform on [0, 2 pi)
-    std::vector<double> cdf(M.nf);
-    double tot = 0;
-    for (int f = 0; f < M.nf; ++f) {
-      const double* A = &M.xyz[3 * M.faces[3 * f]];
-      const double* B = &M.xyz[3 * M.faces[3 * f + 1]];
-      const double* C = &M.xyz[3 * M.faces[3 * f + 2]];
-      double e1[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]}, e2[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
-      double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2], cz = e1[0] * e2[1] - e1[1] * e2[0];
-      tot += 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
-      cdf[f] = tot;
-    }
     const int n = s->n;
     std::vector<float> x(n), y(n), th(n);
-    for (int i = 0; i < n; ++i) {
-      const tissue::Draw4 d = tissue::philox_draw(seed, (uint32_t)i, 0u, 0u);
-      const double u0 = tissue::u53(d.w[0], d.w[1]);
-      int f = (int)(std::upper_bound(cdf.begin(), cdf.end(), u0 * tot) - cdf.begin());
-      f = std::min(f, M.nf - 1);
-      const tissue::Draw4 e = tissue::philox_draw(seed, (uint32_t)i, 0u, 2u);
-      double r1 = tissue::u53(d.w[2], d.w[3]), r2 = tissue::u53(e.w[0], e.w[1]);
-      if (r1 + r2 > 1.0) { r1 = 1.0 - r1; r2 = 1.0 - r2; }
-      const double* a = &M.uv[2 * M.faces[3 * f]];
-      const double* b = &M.uv[2 * M.faces[3 * f + 1]];
-      const double* c = &M.uv[2 * M.faces[3 * f + 2]];
-      x[i] = (float)std::clamp(a[0] + r1 * (b[0] - a[0]) + r2 * (c[0] - a[0]), 0.0, 1.0);
-      y[i] = (float)std::clamp(a[1] + r1 * (b[1] - a[1]) + r2 * (c[1] - a[1]), 0.0, 1.0);
-      double t = 2.0 * M_PI * tissue::u53(e.w[2], e.w[3]);
-      t = t - 2.0 * M_PI * std::floor((t + M_PI) / (2.0 * M_PI));
-      th[i] = (float)t;
-    }
+    random_init(s->mesh->m, seed, n, x.data(), y.data(), th.data());
     return tissue_set_state(s, x.data(), y.data(), th.data(), nullptr);
   });
 }
@@ -695,6 +597,10 @@ int32_t tissue_neighbor_lists(tissue_sim* s, int32_t* counts, int32_t* cols, int
     int* dc = dalloc<int>(n);
     int* dl = dalloc<int>((size_t)n * cap);
     try {
+      if (s->buckets_stale) {
+        rebuild_buckets(s);
+        s->buckets_stale = false;
+      }
       HIP_OK(tissue::launch_neighbor_lists(s->md, s->sp, n, s->bin(s->bphase), s->vid[s->sphase], cap, dc, dl, s->stream));
       HIP_OK(hipMemcpyAsync(counts, dc, n * sizeof(int), hipMemcpyDeviceToHost, s->stream));
       HIP_OK(hipMemcpyAsync(cols, dl, (size_t)n * cap * sizeof(int), hipMemcpyDeviceToHost, s->stream));
@@ -709,12 +615,33 @@ int32_t tissue_neighbor_lists(tissue_sim* s, int32_t* counts, int32_t* cols, int
   });
 }
 
+int32_t tissue_set_mode(tissue_sim* s, int32_t mode) {
+  return guarded([&] {
+    if (!s) throw ApiError(TISSUE_E_INVALID_ARG, "null simulation handle");
+    if (mode != TISSUE_MODE_AUTO && mode != TISSUE_MODE_STEP_KERNEL && mode != TISSUE_MODE_RESIDENT)
+      throw ApiError(TISSUE_E_INVALID_ARG, "unknown mode");
+    if (mode == TISSUE_MODE_RESIDENT && s->n > s->resident_cap)
+      throw ApiError(TISSUE_E_INVALID_ARG, "too many particles for the resident kernel on this mesh");
+    s->mode = mode;
+    return TISSUE_OK;
+  });
+}
+
+int32_t tissue_resident_capacity(const tissue_mesh* m, int32_t* max_particles) {
+  return guarded([&] {
+    if (!m || !max_particles) throw ApiError(TISSUE_E_INVALID_ARG, "null argument");
+    *max_particles = tissue::resident_max_particles(m->m.nv);
+    return TISSUE_OK;
+  });
+}
+
 int32_t tissue_set_launch(tissue_sim* s, int32_t lanes, int32_t block) {
   return guarded([&] {
     if (!s) throw ApiError(TISSUE_E_INVALID_ARG, "null simulation handle");
     if (lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16)
       throw ApiError(TISSUE_E_INVALID_ARG, "lanes must be 0, 1, 2, 4, 8 or 16");
-    if (block != 0 && (block < 64 || block > 1024 || block % 64)) throw ApiError(TISSUE_E_INVALID_ARG, "block must be a multiple of 64 in [64, 1024]");
+    if (block != 0 && (block < 64 || block > tissue::kStepMaxBlock || block % 64))
+      throw ApiError(TISSUE_E_INVALID_ARG, "block must be a multiple of 64 in [64, 512]");
     s->lanes = lanes;
     s->block = block;
     return TISSUE_OK;

@@ -1,0 +1,338 @@
+// Geometry of the step that decides discrete outcomes: the seam wrap and the UV -> 3-D point
+// location. Written as __host__ __device__ so that the very same source is compiled for the host
+// (tests/hostcheck) and checked decision for decision against the FP64 oracle.
+//
+// Both parts follow the reference's FP64 arithmetic as written (compiled with -ffp-contract=off)
+// and only skip work whose outcome is already determined:
+//  - a border test t = n_t / det in [0, 1] (and likewise s) holds exactly when n_t == 0 or n_t
+//    and det have the same sign, and |n_t| <= |det|: for doubles, |n| > |d| implies
+//    fl(|n| / |d|) > 1 and |n| <= |d| implies fl(|n| / |d|) <= 1, so the division is only done for
+//    the border that is crossed;
+//  - point location first ranks the candidate faces with FP32 distances and keeps that ranking
+//    only when the best two differ by more than kLocateMargin, far above the FP32 rounding of a
+//    distance between points of [0, 1]^2; otherwise the FP64 distances decide. The nearest-vertex
+//    choice is screened the same way in 3-D.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tdev {
+
+constexpr double kPi = 3.141592653589793;
+constexpr float kLocateMargin = 1e-5f;   // UV units (coordinates in [0, 1])
+constexpr float kVertexMargin = 1e-4f;   // 3-D units, relative to the mesh's extent (|X| <= ~10)
+constexpr int kFaceChunk = 4;            // candidate faces per lane loaded together
+
+struct MeshDev {
+  int32_t nv, nf, grid, pad;
+  const float2* uv;        // [nv]
+  const float* xyz;        // [nv*3]
+  const int32_t* faces;    // [nf*3]
+  const int32_t* cell_start;
+  const int32_t* cell_faces;
+  const int32_t* row_ptr;  // [nv+1]
+  const int2* nbr;         // (vertex, float bits of the geodesic distance), rows ascending in distance
+};
+
+// ------------------------------------------------------------------ seam wrap (FP64)
+
+__host__ __device__ inline bool inside_unit(double x, double y) { return x >= 0.0 && x <= 1.0 && y >= 0.0 && y <= 1.0; }
+
+// fl(n / d) in [0, 1], decided without dividing (d != 0)
+__host__ __device__ inline bool quotient_in_unit(double n, double d) {
+  return (n == 0.0 || ((n > 0.0) == (d > 0.0))) && fabs(n) <= fabs(d);
+}
+
+// EuclideanTiling::intersection_point (:170-208) of the step segment A -> A + (dx, dy) with each
+// border of the unit square, the border's constant edge vector folded in: for C = (0,0),
+// D = (0,1), det = (bx-ax)*1 - (by-ay)*0 = bx-ax exactly, and so on. Every rounding happens in the
+// reference's operation order, so the results equal the general formula's bit for bit
+// (tests/test_device_math.py checks this against the oracle). is_point_on_segment (:156-168)
+// reduces to "A lies on this edge"; |det| < 1e-9 rejects the border, as there.
+__host__ __device__ inline bool cross_left(double ax, double ay, double dx, double dy, double& ox, double& oy) {
+  if (ax == 0.0 && ay >= 0.0 && ay <= 1.0) return false;  // start on this border
+  const double det = dx;
+  if (fabs(det) < 1e-9) return false;
+  const double nt = -ax, ns = (-ax) * dy - (-ay) * dx;
+  if (!quotient_in_unit(nt, det) || !quotient_in_unit(ns, det)) return false;
+  const double t = nt / det;
+  ox = ax + t * dx;
+  oy = ay + t * dy;
+  return true;
+}
+__host__ __device__ inline bool cross_right(double ax, double ay, double dx, double dy, double& ox, double& oy) {
+  if (ax == 1.0 && ay >= 0.0 && ay <= 1.0) return false;
+  const double det = dx;
+  if (fabs(det) < 1e-9) return false;
+  const double cx = 1.0 - ax;
+  const double nt = cx, ns = cx * dy - (-ay) * dx;
+  if (!quotient_in_unit(nt, det) || !quotient_in_unit(ns, det)) return false;
+  const double t = nt / det;
+  ox = ax + t * dx;
+  oy = ay + t * dy;
+  return true;
+}
+__host__ __device__ inline bool cross_up(double ax, double ay, double dx, double dy, double& ox, double& oy) {
+  if (ay == 1.0 && ax >= 0.0 && ax <= 1.0) return false;
+  const double det = -dy;
+  if (fabs(det) < 1e-9) return false;
+  const double cy = 1.0 - ay;
+  const double nt = -cy, ns = (-ax) * dy - cy * dx;
+  if (!quotient_in_unit(nt, det) || !quotient_in_unit(ns, det)) return false;
+  const double t = nt / det;
+  ox = ax + t * dx;
+  oy = ay + t * dy;
+  return true;
+}
+__host__ __device__ inline bool cross_down(double ax, double ay, double dx, double dy, double& ox, double& oy) {
+  if (ay == 0.0 && ax >= 0.0 && ax <= 1.0) return false;
+  const double det = -dy;
+  if (fabs(det) < 1e-9) return false;
+  const double nt = ay, ns = (-ax) * dy - (-ay) * dx;
+  if (!quotient_in_unit(nt, det) || !quotient_in_unit(ns, det)) return false;
+  const double t = nt / det;
+  ox = ax + t * dx;
+  oy = ay + t * dy;
+  return true;
+}
+
+// Seam wrap of one particle: the reference's restart loop (EuclideanTiling.cpp:31-69) acts on
+// each particle independently, so it runs here as a per-particle loop. (ax, ay) -> (px, py) is the
+// step's segment; on return (px, py) is inside the unit square (or the crossing bound was hit)
+// and dtheta holds the heading rotation (-pi/2 per left/right, -3 pi/2 per up/down crossing).
+// Returns false when more than max_crossings crossings would be needed.
+__host__ __device__ inline bool seam_wrap(double ax, double ay, double& px, double& py, double& dtheta,
+                                          int max_crossings, int* crossings_out = nullptr) {
+  int crossings = 0;
+  dtheta = 0.0;
+  bool ok = true;
+  while (!inside_unit(px, py)) {
+    if (crossings >= max_crossings) { ok = false; break; }
+    ++crossings;
+    // check_border_crossings (:71-101): left, right, up, down; exit snaps to 0 below 1e-3
+    double ex = ax, ey = ay;
+    int border = 4;
+    const double dx = px - ax, dy = py - ay;
+    if (cross_left(ax, ay, dx, dy, ex, ey)) border = 0;
+    else if (cross_right(ax, ay, dx, dy, ex, ey)) border = 1;
+    else if (cross_up(ax, ay, dx, dy, ex, ey)) border = 2;
+    else if (cross_down(ax, ay, dx, dy, ex, ey)) border = 3;
+    if (border < 4) {
+      if (fabs(ex) < 1e-3) ex = 0.0;
+      if (fabs(ey) < 1e-3) ey = 0.0;
+    }
+    // processPoints (:112-154)
+    const double ox = px, oy = py;
+    if (border == 0) { px = oy; py = -ox; dtheta -= 0.5 * kPi; }
+    else if (border == 1) { px = oy; py = 2 - ox; dtheta -= 0.5 * kPi; }
+    else if (border == 2) { px = 2 - oy; py = ox; dtheta -= 1.5 * kPi; }
+    else { px = -oy; py = ox; dtheta -= 1.5 * kPi; }
+    ax = ey;  // entry point = swapped exit point
+    ay = ex;
+  }
+  if (crossings_out) *crossings_out = crossings;
+  return ok;
+}
+
+// heading kept in [-pi, pi)
+__host__ __device__ inline double wrap_heading(double t) { return t - 2.0 * kPi * floor((t + kPi) / (2.0 * kPi)); }
+
+// ------------------------------------------------------------------ point location
+
+template <class T>
+__host__ __device__ inline T sqr(T a) { return a * a; }
+
+// CellHelper::pointSegmentDistance (CellHelper.cpp:216-222), z = 0
+template <class T>
+__host__ __device__ inline T point_segment_distance(T px, T py, T ax, T ay, T bx, T by) {
+  const T abx = bx - ax, aby = by - ay;
+  T t = (abx * (px - ax) + aby * (py - ay)) / (abx * abx + aby * aby);
+  t = t < T(0) ? T(0) : (T(1) < t ? T(1) : t);
+  const T qx = ax + abx * t - px, qy = ay + aby * t - py;
+  return sqrt(qx * qx + qy * qy);
+}
+
+// CellHelper::pointTriangleDistance (CellHelper.cpp:162-214), z = 0
+template <class T>
+__host__ __device__ inline T point_triangle_distance(T px, T py, T ax, T ay, T bx, T by, T cx, T cy) {
+  const T abx = bx - ax, aby = by - ay;
+  const T acx = cx - ax, acy = cy - ay;
+  const T apx = px - ax, apy = py - ay;
+  const T bpx = px - bx, bpy = py - by;
+  const T cpx = px - cx, cpy = py - cy;
+  const T d_ab_ap = abx * apx + aby * apy;
+  const T d_ac_ap = acx * apx + acy * apy;
+  const T d_ab_bp = abx * bpx + aby * bpy;
+  const T d_ac_bp = acx * bpx + acy * bpy;
+  const T d_ab_cp = abx * cpx + aby * cpy;
+  const T d_ac_cp = acx * cpx + acy * cpy;
+  if (d_ab_ap <= T(0) && d_ac_ap <= T(0)) return sqrt(apx * apx + apy * apy);
+  if (d_ab_bp >= T(0) && d_ac_bp <= d_ab_bp) return sqrt(bpx * bpx + bpy * bpy);
+  if (d_ac_cp >= T(0) && d_ab_cp <= d_ac_cp) return sqrt(cpx * cpx + cpy * cpy);
+  const T vc = d_ab_ap * d_ac_bp - d_ab_bp * d_ac_ap;
+  if (vc <= T(0) && d_ab_ap >= T(0) && d_ab_bp <= T(0)) return point_segment_distance(px, py, ax, ay, bx, by);
+  const T vb = d_ab_cp * d_ac_ap - d_ab_ap * d_ac_cp;
+  if (vb <= T(0) && d_ac_ap >= T(0) && d_ac_cp <= T(0)) return point_segment_distance(px, py, ax, ay, cx, cy);
+  const T va = d_ab_bp * d_ac_cp - d_ab_cp * d_ac_bp;
+  if (va <= T(0) && (d_ac_bp - d_ab_bp) >= T(0) && (d_ab_cp - d_ac_cp) >= T(0))
+    return point_segment_distance(px, py, bx, by, cx, cy);
+  const T denom = T(1) / (va + vb + vc);
+  const T v = vb * denom;
+  const T w = vc * denom;
+  const T qx = ax + abx * v + acx * w - px;
+  const T qy = ay + aby * v + acy * w - py;
+  return sqrt(qx * qx + qy * qy);
+}
+
+struct Located {
+  int face, vid;
+  double X, Y, Z;
+  int fp64;  // 1: the face ranking needed FP64, 2: the vertex choice needed FP64 (bit mask)
+};
+
+__host__ __device__ inline bool lex_less(double d, int f, double bd, int bf) { return d < bd || (d == bd && f < bf); }
+
+// combine the (best, runner-up) distances of two lanes
+__host__ __device__ inline void merge_best(float& b1, int& f1, float& b2, float o1, int of1, float o2) {
+  if (o1 < b1 || (o1 == b1 && of1 < f1)) {
+    b2 = fminf(b1, o2);
+    b1 = o1;
+    f1 = of1;
+  } else {
+    b2 = fminf(b2, o1);
+  }
+}
+
+template <int LANES, class T>
+__host__ __device__ inline T lane_xor(T v, int off) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __shfl_xor(v, off, LANES);
+#else
+  (void)off;
+  return v;
+#endif
+}
+
+// CellHelper::calculate_barycentric_3D_coord (CellHelper.cpp:104-160) with the candidate faces
+// taken from the face bucket of the point's grid cell. Faces containing the point (distance 0
+// up to rounding) always share its cell, so the lexicographic (distance, face) minimum over the
+// bucket is the minimum over all faces. LANES lanes split the bucket and reduce the minimum.
+template <int LANES>
+__host__ __device__ inline Located locate(const MeshDev& m, double px, double py, int lane) {
+  const int G = m.grid;
+  const double gx = px * G, gy = py * G;
+  const int cx = gx >= (double)G ? G - 1 : (gx >= 1.0 ? (int)gx : 0);
+  const int cy = gy >= (double)G ? G - 1 : (gy >= 1.0 ? (int)gy : 0);
+  const int cell = cy * G + cx;
+  const int cs = m.cell_start[cell], ce = m.cell_start[cell + 1];
+  Located r;
+  r.fp64 = 0;
+  // FP32 ranking: best and runner-up distance
+  const float qx = (float)px, qy = (float)py;
+  float b1 = __builtin_inff(), b2 = __builtin_inff();
+  int f1 = 0x7fffffff;
+  // candidates in chunks of kFaceChunk per lane: face ids, then their corners, then the corners'
+  // UV coordinates are loaded together (same visiting order as one at a time)
+  for (int q0 = cs + lane; q0 < ce; q0 += kFaceChunk * LANES) {
+    int f[kFaceChunk], ia[kFaceChunk], ib[kFaceChunk], ic[kFaceChunk];
+    float2 a[kFaceChunk], b[kFaceChunk], c[kFaceChunk];
+#pragma unroll
+    for (int k = 0; k < kFaceChunk; ++k) {
+      const int q = q0 + k * LANES;
+      f[k] = q < ce ? m.cell_faces[q] : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < kFaceChunk; ++k)
+      if (f[k] >= 0) { ia[k] = m.faces[3 * f[k]]; ib[k] = m.faces[3 * f[k] + 1]; ic[k] = m.faces[3 * f[k] + 2]; }
+#pragma unroll
+    for (int k = 0; k < kFaceChunk; ++k)
+      if (f[k] >= 0) { a[k] = m.uv[ia[k]]; b[k] = m.uv[ib[k]]; c[k] = m.uv[ic[k]]; }
+#pragma unroll
+    for (int k = 0; k < kFaceChunk; ++k)
+      if (f[k] >= 0) {
+        const float d = point_triangle_distance<float>(qx, qy, a[k].x, a[k].y, b[k].x, b[k].y, c[k].x, c[k].y);
+        merge_best(b1, f1, b2, d, f[k], __builtin_inff());
+      }
+  }
+  if constexpr (LANES > 1) {
+#pragma unroll
+    for (int off = LANES / 2; off > 0; off >>= 1) {
+      const float o1 = lane_xor<LANES>(b1, off), o2 = lane_xor<LANES>(b2, off);
+      const int of1 = lane_xor<LANES>(f1, off);
+      merge_best(b1, f1, b2, o1, of1, o2);
+    }
+  }
+  int best_f = f1;
+  if (!(b2 - b1 > kLocateMargin)) {  // close call (or non-finite input): rank in FP64
+    r.fp64 |= 1;
+    double bd = __builtin_inf();
+    int bf = 0x7fffffff;
+    for (int q = cs + lane; q < ce; q += LANES) {
+      const int f = m.cell_faces[q];
+      const float2 a = m.uv[m.faces[3 * f]], b = m.uv[m.faces[3 * f + 1]], c = m.uv[m.faces[3 * f + 2]];
+      const double d = point_triangle_distance<double>(px, py, a.x, a.y, b.x, b.y, c.x, c.y);
+      if (lex_less(d, f, bd, bf)) { bd = d; bf = f; }
+    }
+    if constexpr (LANES > 1) {
+#pragma unroll
+      for (int off = LANES / 2; off > 0; off >>= 1) {
+        const double od = lane_xor<LANES>(bd, off);
+        const int of = lane_xor<LANES>(bf, off);
+        if (lex_less(od, of, bd, bf)) { bd = od; bf = of; }
+      }
+    }
+    best_f = bf;
+  }
+  if (best_f == 0x7fffffff) best_f = 0;  // non-finite point: any valid face keeps indices in range
+  const int ia = m.faces[3 * best_f], ib = m.faces[3 * best_f + 1], ic = m.faces[3 * best_f + 2];
+  const float2 ua = m.uv[ia], ub = m.uv[ib], uc = m.uv[ic];
+  // FP32 blend and nearest-vertex choice, kept when the choice is clear
+  {
+    float wa = sqrtf(sqr(qx - ua.x) + sqr(qy - ua.y));
+    float wb = sqrtf(sqr(qx - ub.x) + sqr(qy - ub.y));
+    float wc = sqrtf(sqr(qx - uc.x) + sqr(qy - uc.y));
+    const float sw = wa + wb + wc;
+    wa /= sw; wb /= sw; wc /= sw;
+    const float Ax = m.xyz[3 * ia], Ay = m.xyz[3 * ia + 1], Az = m.xyz[3 * ia + 2];
+    const float Bx = m.xyz[3 * ib], By = m.xyz[3 * ib + 1], Bz = m.xyz[3 * ib + 2];
+    const float Cx = m.xyz[3 * ic], Cy = m.xyz[3 * ic + 1], Cz = m.xyz[3 * ic + 2];
+    const float X = wa * Ax + wb * Bx + wc * Cx, Y = wa * Ay + wb * By + wc * Cy, Z = wa * Az + wb * Bz + wc * Cz;
+    const float da = sqrtf(sqr(X - Ax) + sqr(Y - Ay) + sqr(Z - Az));
+    const float db = sqrtf(sqr(X - Bx) + sqr(Y - By) + sqr(Z - Bz));
+    const float dc = sqrtf(sqr(X - Cx) + sqr(Y - Cy) + sqr(Z - Cz));
+    // gap between the smallest distance and the other two
+    const float mn = fminf(da, fminf(db, dc));
+    const float gap = (da == mn ? fminf(db, dc) : (db == mn ? fminf(da, dc) : fminf(da, db))) - mn;
+    if (gap > kVertexMargin && (da == mn) + (db == mn) + (dc == mn) == 1) {
+      r.X = X; r.Y = Y; r.Z = Z;
+      r.vid = da == mn ? ia : (db == mn ? ib : ic);
+      r.face = best_f;
+      return r;
+    }
+  }
+  r.fp64 |= 2;
+  const double Ax = m.xyz[3 * ia], Ay = m.xyz[3 * ia + 1], Az = m.xyz[3 * ia + 2];
+  const double Bx = m.xyz[3 * ib], By = m.xyz[3 * ib + 1], Bz = m.xyz[3 * ib + 2];
+  const double Cx = m.xyz[3 * ic], Cy = m.xyz[3 * ic + 1], Cz = m.xyz[3 * ic + 2];
+  double w_a = sqrt(sqr(px - (double)ua.x) + sqr(py - (double)ua.y));
+  double w_b = sqrt(sqr(px - (double)ub.x) + sqr(py - (double)ub.y));
+  double w_c = sqrt(sqr(px - (double)uc.x) + sqr(py - (double)uc.y));
+  const double sum_weights = w_a + w_b + w_c;  // CellHelper::normalize_weights (:224-230)
+  w_a /= sum_weights;
+  w_b /= sum_weights;
+  w_c /= sum_weights;
+  r.X = w_a * Ax + w_b * Bx + w_c * Cx;
+  r.Y = w_a * Ay + w_b * By + w_c * Cy;
+  r.Z = w_a * Az + w_b * Bz + w_c * Cz;
+  const double da = sqrt(sqr(r.X - Ax) + sqr(r.Y - Ay) + sqr(r.Z - Az));
+  const double db = sqrt(sqr(r.X - Bx) + sqr(r.Y - By) + sqr(r.Z - Bz));
+  const double dc = sqrt(sqr(r.X - Cx) + sqr(r.Y - Cy) + sqr(r.Z - Cz));
+  double mn = da;
+  if (db < mn) mn = db;
+  if (dc < mn) mn = dc;
+  r.vid = (mn == da) ? ia : ((mn == db) ? ib : ic);
+  r.face = best_f;
+  return r;
+}
+
+}  // namespace tdev

@@ -49,7 +49,7 @@ __device__ void block_order_partial(int sgn, double cs, double sn, OrderPartial*
 }
 
 template <int LANES>
-__global__ void __launch_bounds__(1024) k_step(MeshDev mesh, StepParams prm, int n, BucketsIn bin, BucketsOut bout,
+__global__ void __launch_bounds__(kStepMaxBlock) k_step(MeshDev mesh, StepParams prm, int n, BucketsIn bin, BucketsOut bout,
                                                int* clear_count, int* clear_ovf_n, StepIO io) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & (LANES - 1);

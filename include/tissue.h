@@ -106,11 +106,48 @@ int32_t tissue_get_flags(tissue_sim* s, uint32_t* flags, int32_t* first_step);
 /* neighbour sets {j : d_ij < 2 sigma} (self included) of the current state, at most cap per
  * particle: counts[n], cols[n*cap], each list ascending */
 int32_t tissue_neighbor_lists(tissue_sim* s, int32_t* counts, int32_t* cols, int32_t cap);
-/* launch shape: lanes per particle (1, 2, 4, 8, 16) and threads per block (64..1024); 0 = auto */
+/* launch shape: lanes per particle (1, 2, 4, 8, 16) and threads per block (64..512); 0 = auto */
 int32_t tissue_set_launch(tissue_sim* s, int32_t lanes, int32_t block);
+/* execution mode of tissue_step*: one kernel launch per step over the whole GPU, or the
+ * persistent LDS-resident kernel (one workgroup, many steps per launch; needs
+ * n <= tissue_resident_capacity). AUTO picks the resident kernel whenever it fits. */
+#define TISSUE_MODE_AUTO 0
+#define TISSUE_MODE_STEP_KERNEL 1
+#define TISSUE_MODE_RESIDENT 2
+int32_t tissue_set_mode(tissue_sim* s, int32_t mode);
+int32_t tissue_resident_capacity(const tissue_mesh* m, int32_t* max_particles);
 int32_t tissue_synchronize(tissue_sim* s);
 /* GPU time of the step kernels of the last tissue_step call (HIP events on the handle's stream) */
 int32_t tissue_last_step_ms(tissue_sim* s, double* ms);
+
+/* ---- many independent replicas in one persistent launch ----
+ * Replica r owns particles [off_r, off_r + n[r]) of the concatenated arrays (off_r = sum of the
+ * earlier n). Noise rows are [steps][sum n] with replica r's columns at off_r. One workgroup runs
+ * one replica with its state resident in LDS; the launches of a run are captured in a hipGraph
+ * and replayed. A replica that raises a validation flag stops (its later steps are no-ops). */
+typedef struct tissue_batch tissue_batch;
+int32_t tissue_batch_create(const tissue_mesh* m, int32_t n_replicas, const int32_t* n, const tissue_params* params,
+                            tissue_batch** out);
+void tissue_batch_destroy(tissue_batch* b);
+int32_t tissue_batch_init_random(tissue_batch* b, const uint64_t* seeds /* [n_replicas] */);
+int32_t tissue_batch_set_state(tissue_batch* b, const float* x, const float* y, const float* theta, const int32_t* vid);
+int32_t tissue_batch_get_state(tissue_batch* b, float* x, float* y, float* theta, int32_t* vid);
+int32_t tissue_batch_get_outputs(tissue_batch* b, float* vx, float* vy, float* r3d, int32_t* face, int32_t* nbr);
+/* eta for replica r, step t: stds[r] * N(0,1) of the stream seeds[r] (as tissue_gen_noise) */
+int32_t tissue_batch_stage_noise(tissue_batch* b, const float* noise, int32_t n_rows);
+int32_t tissue_batch_gen_noise(tissue_batch* b, const uint64_t* seeds, const double* stds, int32_t n_rows);
+/* n_steps steps from staged noise row first_row (first_row < 0: no noise) */
+int32_t tissue_batch_run(tissue_batch* b, int32_t n_steps, int32_t first_row);
+int32_t tissue_batch_get_order(tissue_batch* b, int32_t replica, double* phi_ref, double* phi_polar, int32_t cap,
+                               int32_t* count);
+int32_t tissue_batch_get_flags(tissue_batch* b, uint32_t* flags /* [n_replicas] */, int32_t* first_step);
+/* lanes per particle (1, 2, 4, 8), threads per workgroup (64..512), steps per launch; 0 = auto */
+int32_t tissue_batch_set_launch(tissue_batch* b, int32_t lanes, int32_t block, int32_t steps_per_launch);
+/* time n_steps of every launch shape that fits (state restored after each trial) and keep the
+ * fastest; results are cached per (max replica size, mesh, sigma) for the process */
+int32_t tissue_batch_autotune(tissue_batch* b, int32_t n_steps, int32_t* lanes, int32_t* block);
+/* GPU time of the last tissue_batch_run (HIP events around the graph launch) */
+int32_t tissue_batch_last_run_ms(tissue_batch* b, double* ms);
 
 /* ---- host helpers (no GPU) ---- */
 /* eta[t][i] = std * N(0,1) from Philox4x32-10 (Salmon et al., SC'11) + Box-Muller */

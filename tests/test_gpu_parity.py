@@ -1,5 +1,6 @@
 """GPU parity: the HIP step (through the C-ABI) against the FP64 oracle, teacher-forced: every
-step of the GPU trajectory is recomputed by the oracle from the GPU's own input state."""
+step of the GPU trajectory is recomputed by the oracle from the GPU's own input state. Both
+execution modes are covered: one kernel launch per step, and the persistent LDS-resident kernel."""
 import numpy as np
 import pytest
 
@@ -8,11 +9,12 @@ import parity as P
 pytestmark = pytest.mark.gpu
 
 
-def run_teacher_forced(T, O, cfg, steps, lanes=0):
+def run_teacher_forced(T, O, cfg, steps, lanes=0, mode=0):
     m, tb = P.mesh_and_tables(cfg["freq"], cfg["sigma"])
     om = O.Mesh(tb)
     p = dict(P.PHYS, sigma=cfg["sigma"])
     sim = T.Sim(m, T.params(**p), cfg["n"])
+    sim.set_mode(mode)
     sim.set_launch(lanes, 0)
     sim.init_random(cfg["seed"])
     noise = T.gen_noise(cfg["noise_seed"], 0.1, steps, cfg["n"])
@@ -31,12 +33,14 @@ def run_teacher_forced(T, O, cfg, steps, lanes=0):
     return total
 
 
+@pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("lanes", [1, 8])
-def test_c1_teacher_forced_50_steps(T, O, lanes):
-    total = run_teacher_forced(T, O, P.C1, 50, lanes)
+def test_c1_teacher_forced_50_steps(T, O, lanes, mode):
+    total = run_teacher_forced(T, O, P.C1, 50, lanes, mode)
     assert all(v == 0 for v in total.values()), total
 
 
-def test_c2_teacher_forced_20_steps(T, O):
-    total = run_teacher_forced(T, O, P.C2, 20)
+@pytest.mark.parametrize("mode", [1, 2])
+def test_c2_teacher_forced_20_steps(T, O, mode):
+    total = run_teacher_forced(T, O, P.C2, 20, 0, mode)
     assert all(v == 0 for v in total.values()), total

@@ -1,0 +1,186 @@
+"""GPU tests of the persistent LDS-resident kernel: single simulations (resident mode) and
+batches of replicas. Both kernels share the per-particle update and add neighbours in the same
+order, so with the same lanes per particle their results are bitwise equal; the batch must
+reproduce each replica run alone."""
+import numpy as np
+import pytest
+
+import parity as P
+
+pytestmark = pytest.mark.gpu
+
+
+def make_sim(T, m, cfg, mode, lanes, block=0):
+    p = dict(P.PHYS, sigma=cfg["sigma"])
+    s = T.Sim(m, T.params(**p), cfg["n"])
+    s.set_mode(mode)
+    s.set_launch(lanes, block)
+    s.init_random(cfg["seed"])
+    return s
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 8])
+def test_resident_equals_step_kernel(T, lanes):
+    cfg = P.C2
+    m, _ = P.mesh_and_tables(cfg["freq"], cfg["sigma"])
+    noise = T.gen_noise(cfg["noise_seed"], 0.1, 30, cfg["n"])
+    a = make_sim(T, m, cfg, T.MODE_STEP_KERNEL, lanes)
+    b = make_sim(T, m, cfg, T.MODE_RESIDENT, lanes)
+    assert a.step(30, noise) == T.OK
+    assert b.step(30, noise) == T.OK
+    sa, sb = a.get_state(), b.get_state()
+    for k in ("x", "y", "theta", "vid"):
+        assert np.array_equal(getattr(sa, k), getattr(sb, k)), k
+    oa, ob = a.get_outputs(), b.get_outputs()
+    for k in ("vx", "vy", "r3d", "face", "nbr"):
+        assert np.array_equal(getattr(oa, k), getattr(ob, k)), k
+    (ra, pa), (rb, pb) = a.order(), b.order()
+    assert np.array_equal(ra, rb)
+    assert np.allclose(pa, pb, rtol=0, atol=1e-12)
+
+
+def test_resident_teacher_forced_c1(T, O):
+    cfg = P.C1
+    m, tb = P.mesh_and_tables(cfg["freq"], cfg["sigma"])
+    om = O.Mesh(tb)
+    p = dict(P.PHYS, sigma=cfg["sigma"])
+    sim = make_sim(T, m, cfg, T.MODE_RESIDENT, 0)
+    noise = T.gen_noise(cfg["noise_seed"], 0.1, 50, cfg["n"])
+    total = {}
+    for t in range(50):
+        s_in = sim.get_state()
+        nb = sim.neighbor_lists()
+        assert sim.step(1, noise[t]) == T.OK
+        fails, _ = P.compare_step(O, om, p, s_in, sim.get_state(), sim.get_outputs(), nb,
+                                  noise[t].astype(np.float64))
+        for k, v in fails.items():
+            total[k] = total.get(k, 0) + v
+    assert all(v == 0 for v in total.values()), total
+
+
+BATCH_SIZES = [100, 517, 1000, 2000, 3100, 4000]
+
+
+def batch_params(T, sigma):
+    rng = np.random.default_rng(3)
+    R = len(BATCH_SIZES)
+    tau = np.exp(rng.uniform(np.log(0.1), np.log(10.0), R))
+    std = rng.uniform(0.0, 0.5, R)
+    v0 = rng.uniform(0.5, 2.0, R)
+    ps = [T.params(sigma=sigma, v0=float(v0[r]), k=10.0, mu=1.0, tau=float(tau[r]), dt=0.02) for r in range(R)]
+    return ps, std
+
+
+def test_batch_equals_solo_runs(T):
+    sigma = 0.05
+    m, _ = P.mesh_and_tables(32, sigma)
+    ps, std = batch_params(T, sigma)
+    seeds = np.array([1000 + r for r in range(len(BATCH_SIZES))], np.uint64)
+    steps = 40
+    b = T.Batch(m, BATCH_SIZES, ps)
+    b.set_launch(1, 512, 16)  # several launches per run
+    b.init_random(seeds)
+    b.gen_noise(seeds, std, steps)
+    assert b.run(steps, 0) == T.OK
+    st = b.get_state()
+    out = b.get_outputs()
+    for r, n in enumerate(BATCH_SIZES):
+        o = int(b.off[r])
+        s = T.Sim(m, ps[r], n)
+        s.set_mode(T.MODE_RESIDENT)
+        s.set_launch(1, 512)
+        s.init_random(int(seeds[r]))
+        assert s.step(steps, T.gen_noise(int(seeds[r]), float(std[r]), steps, n)) == T.OK
+        ss = s.get_state()
+        for k in ("x", "y", "theta", "vid"):
+            assert np.array_equal(getattr(ss, k), getattr(st, k)[o:o + n]), (r, k)
+        so = s.get_outputs()
+        for k in ("vx", "vy", "face", "nbr"):
+            assert np.array_equal(getattr(so, k), getattr(out, k)[o:o + n]), (r, k)
+        (ra, pa), (rb, pb) = s.order(), b.order(r)
+        assert np.array_equal(ra, rb) and np.allclose(pa, pb, rtol=0, atol=1e-12), r
+        s.close()
+
+
+def test_batch_autotune_restores_state(T):
+    sigma = 0.05
+    m, _ = P.mesh_and_tables(32, sigma)
+    ps, std = batch_params(T, sigma)
+    seeds = np.arange(len(BATCH_SIZES), dtype=np.uint64) + 7
+    b = T.Batch(m, BATCH_SIZES, ps)
+    b.init_random(seeds)
+    b.gen_noise(seeds, std, 10)
+    before = b.get_state()
+    lanes, block = b.autotune(10)
+    assert lanes in (1, 2, 4, 8) and block in (256, 512)
+    after = b.get_state()
+    assert all(np.array_equal(getattr(before, k), getattr(after, k)) for k in ("x", "y", "theta", "vid"))
+    assert len(b.order(0)[0]) == 0
+    assert b.run(10, 0) == T.OK
+    assert len(b.order(0)[0]) == 10
+
+
+def test_batch_failure_is_per_replica(T):
+    """A replica whose seam wrap exceeds its crossing bound (forced from the host with
+    max_crossings = 1) stops and reports; the other replicas run on exactly as alone."""
+    sigma = 0.05
+    m, _ = P.mesh_and_tables(32, sigma)
+    sizes = [800, 800, 800]
+    ps = [T.params(sigma=sigma, v0=1.0, k=10.0, mu=1.0, tau=1.0, dt=0.02) for _ in sizes]
+    ps[1] = T.params(sigma=sigma, v0=1.0, k=10.0, mu=1.0, tau=1.0, dt=0.02, max_crossings=1)
+    seeds = np.array([11, 12, 13], np.uint64)
+    b = T.Batch(m, sizes, ps)
+    b.set_launch(2, 512, 0)
+    b.init_random(seeds)
+    b.gen_noise(seeds, np.array([0.1, 0.1, 0.1]), 30)
+    code = b.run(30, 0)
+    flags, first = b.flags()
+    assert code == T.E_TILING_NO_CONVERGE
+    assert flags[1] & T.FLAG_TILING and flags[0] == 0 and flags[2] == 0
+    assert 0 <= first[1] < 30
+    s = T.Sim(m, ps[2], 800)
+    s.set_mode(T.MODE_RESIDENT)
+    s.set_launch(2, 512)
+    s.init_random(13)
+    assert s.step(30, T.gen_noise(13, 0.1, 30, 800)) == T.OK
+    st = b.get_state()
+    assert np.array_equal(s.get_state().x, st.x[1600:2400])
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_sim_nonfinite_noise_reported_and_halts(T, mode):
+    cfg = P.C1
+    m, _ = P.mesh_and_tables(cfg["freq"], cfg["sigma"])
+    s = make_sim(T, m, cfg, mode, 0)
+    noise = T.gen_noise(0, 0.1, 5, cfg["n"])
+    noise[2, 17] = np.nan
+    assert s.step(5, noise) == T.E_NONFINITE
+    f, first = s.flags()
+    assert f & T.FLAG_NONFINITE and first == 2
+    # steps after the failure pass the state through: order history of the halted steps is zero
+    r, p = s.order()
+    assert len(r) == 5 and r[3] == 0.0 and r[4] == 0.0
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_sim_crossing_bound_reported(T, mode):
+    cfg = P.C2
+    m, _ = P.mesh_and_tables(cfg["freq"], cfg["sigma"])
+    p = dict(P.PHYS, sigma=cfg["sigma"], max_crossings=1)
+    s = T.Sim(m, T.params(**p), cfg["n"])
+    s.set_mode(mode)
+    s.init_random(cfg["seed"])
+    assert s.step(20, T.gen_noise(1, 0.1, 20, cfg["n"])) == T.E_TILING_NO_CONVERGE
+    f, first = s.flags()
+    assert f & T.FLAG_TILING and first >= 0
+
+
+def test_set_state_rejects_bad_input(T):
+    m, _ = P.mesh_and_tables(22, 0.1)
+    s = T.Sim(m, T.params(sigma=0.1), 3)
+    with pytest.raises(T.TissueError):
+        s.set_state([0.5, 1.5, 0.2], [0.5, 0.5, 0.5], [0, 0, 0])
+    with pytest.raises(T.TissueError):
+        s.set_state([0.5, 0.5, 0.2], [0.5, float("nan"), 0.5], [0, 0, 0])
+    with pytest.raises(T.TissueError):
+        s.step(1)  # no state yet
