@@ -207,8 +207,13 @@ struct tissue_sim {
   bool has_state = false;
   double last_ms = 0.0;
 
+  // AUTO takes the persistent kernel only for systems that one pass of a workgroup covers: beyond
+  // that, each extra pass pays the slowest lane's seam-wrap / point-location chain again, and
+  // the one-launch-per-step kernel (one wave per CU) is faster (N = 2,000: 41 us vs 320 us)
+  static constexpr int kAutoResidentMax = 256;
   bool use_resident() const {
     if (mode == TISSUE_MODE_STEP_KERNEL) return false;
+    if (mode == TISSUE_MODE_AUTO && n > kAutoResidentMax) return false;
     return n <= resident_cap;
   }
   int res_lanes() const { return lanes > 0 ? std::min(lanes, 8) : tissue::resident_default_lanes(n); }

@@ -110,7 +110,8 @@ int32_t tissue_neighbor_lists(tissue_sim* s, int32_t* counts, int32_t* cols, int
 int32_t tissue_set_launch(tissue_sim* s, int32_t lanes, int32_t block);
 /* execution mode of tissue_step*: one kernel launch per step over the whole GPU, or the
  * persistent LDS-resident kernel (one workgroup, many steps per launch; needs
- * n <= tissue_resident_capacity). AUTO picks the resident kernel whenever it fits. */
+ * n <= tissue_resident_capacity). AUTO picks the resident kernel only for n <= 256 (one pass of
+ * a workgroup), the step kernel otherwise. */
 #define TISSUE_MODE_AUTO 0
 #define TISSUE_MODE_STEP_KERNEL 1
 #define TISSUE_MODE_RESIDENT 2
