@@ -87,6 +87,8 @@ SIGNATURES = [
     ("tissue_last_step_ms", C.c_int32, [_P, _D]),
     ("tissue_set_mode", C.c_int32, [_P, C.c_int32]),
     ("tissue_resident_capacity", C.c_int32, [_P, _I]),
+    ("tissue_get_launch", C.c_int32, [_P, _I]),
+    ("tissue_autotune", C.c_int32, [_P, C.c_int32, _I]),
     ("tissue_batch_create", C.c_int32, [_P, C.c_int32, _I, C.POINTER(Params), C.POINTER(_P)]),
     ("tissue_batch_destroy", None, [_P]),
     ("tissue_batch_init_random", C.c_int32, [_P, C.POINTER(C.c_uint64)]),
@@ -99,6 +101,8 @@ SIGNATURES = [
     ("tissue_batch_get_order", C.c_int32, [_P, C.c_int32, _D, _D, C.c_int32, _I]),
     ("tissue_batch_get_flags", C.c_int32, [_P, C.POINTER(C.c_uint32), _I]),
     ("tissue_batch_set_launch", C.c_int32, [_P, C.c_int32, C.c_int32, C.c_int32]),
+    ("tissue_batch_set_mode", C.c_int32, [_P, C.c_int32, C.c_int32]),
+    ("tissue_batch_get_launch", C.c_int32, [_P, _I]),
     ("tissue_batch_autotune", C.c_int32, [_P, C.c_int32, _I, _I]),
     ("tissue_batch_last_run_ms", C.c_int32, [_P, _D]),
     ("tissue_gen_noise", C.c_int32, [C.c_uint64, C.c_double, C.c_int32, C.c_int32, _F]),
@@ -309,6 +313,17 @@ class Sim:
     def set_mode(self, mode: int) -> None:
         _check(lib().tissue_set_mode(self.h, mode))
 
+    def launch(self) -> dict:
+        """Launch shape the next step uses (tissue_get_launch)."""
+        s = np.zeros(5, np.int32)
+        _check(lib().tissue_get_launch(self.h, _ptr(s, C.c_int32)))
+        return _shape_dict(s)
+
+    def autotune(self, n_steps: int) -> dict:
+        s = np.zeros(5, np.int32)
+        _check(lib().tissue_autotune(self.h, n_steps, _ptr(s, C.c_int32)))
+        return _shape_dict(s)
+
     def synchronize(self) -> None:
         _check(lib().tissue_synchronize(self.h))
 
@@ -332,6 +347,10 @@ class Sim:
 MODE_AUTO, MODE_STEP_KERNEL, MODE_RESIDENT = 0, 1, 2
 
 
+def _shape_dict(s: np.ndarray) -> dict:
+    return dict(mode=int(s[0]), lanes=int(s[1]), block=int(s[2]), steps_per_launch=int(s[3]), xcd_order=int(s[4]))
+
+
 def resident_capacity(mesh: Mesh) -> int:
     c = C.c_int32()
     _check(lib().tissue_resident_capacity(mesh.h, C.byref(c)))
@@ -339,7 +358,7 @@ def resident_capacity(mesh: Mesh) -> int:
 
 
 class Batch:
-    """Independent replicas run by the persistent kernel (tissue_batch)."""
+    """Independent replicas run together (tissue_batch): tiled step kernel or persistent kernel."""
 
     def __init__(self, mesh: Mesh, sizes, params_list):
         self.mesh = mesh
@@ -409,6 +428,14 @@ class Batch:
 
     def set_launch(self, lanes: int = 0, block: int = 0, steps_per_launch: int = 0) -> None:
         _check(lib().tissue_batch_set_launch(self.h, lanes, block, steps_per_launch))
+
+    def set_mode(self, mode: int, xcd_order: int = -1) -> None:
+        _check(lib().tissue_batch_set_mode(self.h, mode, xcd_order))
+
+    def launch(self) -> dict:
+        s = np.zeros(5, np.int32)
+        _check(lib().tissue_batch_get_launch(self.h, _ptr(s, C.c_int32)))
+        return _shape_dict(s)
 
     def autotune(self, n_steps: int) -> tuple[int, int]:
         l = C.c_int32(); b = C.c_int32()

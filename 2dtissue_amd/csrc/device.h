@@ -4,8 +4,9 @@
 // Precision plan: state and pair arithmetic in FP32; the decisive or ill-conditioned parts in
 // FP64 (seam-wrap geometry, point location, nearest vertex, order-parameter sign, and the
 // alignment mean when the neighbour sum nearly cancels). All code is compiled with
-// -ffp-contract=off, so every expression rounds as written; the FP64 point-location code uses
-// the same operation order as the host oracle and therefore makes the same decisions.
+// -ffp-contract=off, so every expression rounds as written. Point location ranks candidate faces
+// in FP32 and falls back to FP64 (the host oracle's operation order, so the same decisions)
+// whenever the FP32 ranking or the nearest-vertex choice is within its error margin.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -94,6 +95,10 @@ struct StepOut {
   int nb;                    // neighbour count
   unsigned fl;               // validation flags
   int sgn;                   // sign of x vy - y vx (order parameter)
+  // what the adjoint replays (dead, and removed by the compiler, in kernels that do not record)
+  float Fx, Fy;              // force sum
+  float Sx, Sy;              // alignment sums (cos, sin) over the neighbour set incl. self
+  int q;                     // seam wrap: counter-clockwise quarter turns of the composed map
 };
 
 // table entries per lane loaded together in the neighbour walk
@@ -175,8 +180,10 @@ struct LdsSource {
 // OrientationHelper.cpp:29-74, self included in the mean), count for 0 < d <= 2.4 sigma
 // (2DTissue.cpp:254-268). FP32 sums; FP64 re-evaluation when the displacement is large or the
 // alignment sum nearly cancels.
+// Always inlined: as an out-of-line call (which the inliner picks once two kernels share an
+// instantiation) its result and live state go through scratch memory.
 template <int LANES, class Src>
-__device__ inline StepOut particle_step(const MeshDev& mesh, const StepParams& prm, const Src& src, int i, float4 me,
+__device__ __forceinline__ StepOut particle_step(const MeshDev& mesh, const StepParams& prm, const Src& src, int i, float4 me,
                                         float th, int v, float eta, int lane) {
   float Fx = 0.f, Fy = 0.f, Sx = 0.f, Sy = 0.f;
   int na = 0, nb = 0;
@@ -286,6 +293,8 @@ __device__ inline StepOut particle_step(const MeshDev& mesh, const StepParams& p
       yn = (double)me.y + vy64 * prm.dt64;
       vx = (float)vx64;
       vy = (float)vy64;
+      Fx = (float)Fx64;
+      Fy = (float)Fy64;
     }
     if (weak) {
       Sx64 = group_sum<LANES>(Sx64);
@@ -295,6 +304,8 @@ __device__ inline StepOut particle_step(const MeshDev& mesh, const StepParams& p
       if (delta > kPi) delta -= 2.0 * kPi;
       else if (delta <= -kPi) delta += 2.0 * kPi;
       thn = (double)th + prm.rate64 * delta + (double)eta;
+      Sx = (float)Sx64;
+      Sy = (float)Sy64;
     }
   }
   if (!weak) {
@@ -308,10 +319,15 @@ __device__ inline StepOut particle_step(const MeshDev& mesh, const StepParams& p
   // seam wrap (EuclideanTiling.cpp:31-69) and validation (Validation.cpp:40-72)
   StepOut o;
   o.fl = 0;
+  o.q = 0;
+  o.Fx = Fx;
+  o.Fy = Fy;
+  o.Sx = Sx;
+  o.Sy = Sy;
   double px = xn, py = yn, dth = 0.0;
   const bool finite = isfinite(xn) && isfinite(yn) && isfinite(thn);
   if (!finite) o.fl |= kFlagNonfinite;
-  else if (!seam_wrap(me.x, me.y, px, py, dth, prm.max_crossings)) o.fl |= kFlagTiling;
+  else if (!seam_wrap(me.x, me.y, px, py, dth, prm.max_crossings, nullptr, &o.q)) o.fl |= kFlagTiling;
   if (finite && !inside_unit(px, py)) o.fl |= kFlagLost;
   if (finite) thn = wrap_heading(thn + dth);
   o.xo = (float)px;

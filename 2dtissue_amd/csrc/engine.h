@@ -1,0 +1,111 @@
+// Host engine shared by tissue_sim (one replica) and tissue_batch (R replicas): device state,
+// vertex buckets, launch-shape resolution, hipGraph capture of runs, autotuning, and the
+// bookkeeping of order parameters and validation flags.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "capi_common.h"
+
+namespace tissue {
+
+// launch shape of a run; 0 in a field of a request means "automatic"
+struct Shape {
+  int mode = 0;   // TISSUE_MODE_STEP_KERNEL (tiled, one launch per step) or TISSUE_MODE_RESIDENT
+  int lanes = 0;  // lanes per particle
+  int block = 0;  // threads per workgroup
+  int spl = 0;    // steps per launch (resident) / per order-parameter reduction (tiled)
+  int xcd = -1;   // tiled: 1 = each replica's tiles on one XCD, 0 = linear order, -1 = automatic
+  std::tuple<int, int, int, int, int> key() const { return {mode, lanes, block, spl, xcd}; }
+};
+
+struct Engine {
+  Engine(tissue_mesh* m, int R, const int* n, const tissue_params* p);
+  ~Engine();
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+
+  // state of all replicas, concatenated (replica r at off[r])
+  void set_state(const float* x, const float* y, const float* theta, const int32_t* vid);
+  void init_random(const uint64_t* seeds);
+  void get_state(float* x, float* y, float* theta, int32_t* vid);
+  void get_outputs(float* vx, float* vy, float* r3d, int32_t* face, int32_t* nbr);
+  // neighbour sets {j : d_ij < 2 sigma} of the current state, ids counted within each replica
+  void neighbor_lists(int32_t* counts, int32_t* cols, int32_t cap);
+
+  // noise: rows [rows][ntot] kept on the device for later runs
+  void stage_noise(const float* h, int rows);
+  void gen_noise(const uint64_t* seeds, const double* stds, int rows);
+  const float* staged_rows(int first_row, int n_steps) const;
+  // one call's rows in a separate buffer (leaves the staged rows alone)
+  const float* upload_call_noise(const float* h, int n_steps);
+
+  // n_steps steps; row s of the noise at noise + s * stride (null: no noise). Returns the status.
+  int32_t run(int n_steps, const float* noise, size_t stride);
+  Shape resolve() const;
+  Shape autotune(int n_steps);
+  void set_request(const Shape& s);
+  void synchronize();
+
+  Shape req;
+  std::vector<std::vector<double>> hist_ref, hist_polar;
+  std::vector<unsigned> flags_acc;
+  std::vector<int> first_bad_acc;
+  long steps_done = 0;
+  double last_ms = 0.0;
+
+  tissue_mesh* mesh = nullptr;
+  tdev::MeshDev md{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int R = 0, ntot = 0, nmax = 0, resident_cap = 0;
+  std::vector<int> n, off;
+  std::vector<tissue_params> p;
+  std::vector<ReplicaDesc> reps;
+  bool has_state = false;
+
+  ReplicaDesc* d_reps = nullptr;
+  int* d_rid = nullptr;  // replica of each particle
+  float4* xycs[2] = {nullptr, nullptr};
+  float* th[2] = {nullptr, nullptr};
+  int* vid[2] = {nullptr, nullptr};
+  int sphase = 0;  // state buffer holding the current state
+  float *vx = nullptr, *vy = nullptr, *r3x = nullptr, *r3y = nullptr, *r3z = nullptr;
+  int *face = nullptr, *nbr = nullptr;
+  unsigned* d_halt = nullptr;   // [R] flags of replicas that have stopped (kept across runs)
+  unsigned* d_rows = nullptr;   // [3][R] rotating flag rows of a run
+  int* d_first_bad = nullptr;   // [R]
+
+  // tiled strategy
+  BucketSet bs[3] = {};
+  int bphase = 0;  // bucket set holding the current state's buckets
+  bool buckets_ready = false, buckets_stale = true;
+  TileDesc* d_tiles = nullptr;
+  int* d_tile_begin = nullptr;
+  int n_phys = 0, n_logical = 0;
+  std::tuple<int, int, int> tiles_key{-1, -1, -1};
+  OrderPartial* partial = nullptr;
+  size_t partial_cap = 0;
+
+  double* phi = nullptr;
+  size_t phi_cap = 0;
+  float* noise = nullptr;  // staged rows
+  int noise_rows = 0;
+  float* call_noise = nullptr;
+  size_t call_noise_cap = 0;
+
+  std::map<std::tuple<int, const float*, size_t, std::tuple<int, int, int, int, int>, int, int>, hipGraphExec_t> graphs;
+
+  void drop_graphs();
+  void release();
+  void ensure_buckets();
+  void rebuild_buckets();
+  void ensure_tiles(const Shape& s);
+  void enqueue(const Shape& s, int n_steps, const float* noise, size_t stride);
+};
+
+}  // namespace tissue

@@ -100,10 +100,13 @@ __host__ __device__ inline bool cross_down(double ax, double ay, double dx, doub
 // each particle independently, so it runs here as a per-particle loop. (ax, ay) -> (px, py) is the
 // step's segment; on return (px, py) is inside the unit square (or the crossing bound was hit)
 // and dtheta holds the heading rotation (-pi/2 per left/right, -3 pi/2 per up/down crossing).
-// Returns false when more than max_crossings crossings would be needed.
+// The composed map is p -> R p + b with R a rotation by quarter_out * 90 degrees
+// (counter-clockwise, 0..3); the adjoint needs only R. Returns false when more than
+// max_crossings crossings would be needed.
 __host__ __device__ inline bool seam_wrap(double ax, double ay, double& px, double& py, double& dtheta,
-                                          int max_crossings, int* crossings_out = nullptr) {
+                                          int max_crossings, int* crossings_out = nullptr, int* quarter_out = nullptr) {
   int crossings = 0;
+  int quarter = 0;  // counter-clockwise quarter turns of the composed map's linear part
   dtheta = 0.0;
   bool ok = true;
   while (!inside_unit(px, py)) {
@@ -123,14 +126,15 @@ __host__ __device__ inline bool seam_wrap(double ax, double ay, double& px, doub
     }
     // processPoints (:112-154)
     const double ox = px, oy = py;
-    if (border == 0) { px = oy; py = -ox; dtheta -= 0.5 * kPi; }
-    else if (border == 1) { px = oy; py = 2 - ox; dtheta -= 0.5 * kPi; }
-    else if (border == 2) { px = 2 - oy; py = ox; dtheta -= 1.5 * kPi; }
-    else { px = -oy; py = ox; dtheta -= 1.5 * kPi; }
+    if (border == 0) { px = oy; py = -ox; dtheta -= 0.5 * kPi; --quarter; }
+    else if (border == 1) { px = oy; py = 2 - ox; dtheta -= 0.5 * kPi; --quarter; }
+    else if (border == 2) { px = 2 - oy; py = ox; dtheta -= 1.5 * kPi; ++quarter; }
+    else { px = -oy; py = ox; dtheta -= 1.5 * kPi; ++quarter; }
     ax = ey;  // entry point = swapped exit point
     ay = ex;
   }
   if (crossings_out) *crossings_out = crossings;
+  if (quarter_out) *quarter_out = quarter & 3;
   return ok;
 }
 

@@ -1,13 +1,16 @@
-// gfx950 kernels of the per-step update (one kernel launch per step).
+// gfx950 kernels of the tiled per-step update (one launch per step, all replicas at once).
 //
-// k_step<LANES>: LANES lanes of a wave64 cooperate on one particle. They stride over the
-// particle's row of the truncated geodesic table (vertices within 2.4 sigma of its vertex,
-// ascending in distance), visit the particles bucketed on those vertices, and reduce force,
-// alignment sum and neighbour count with butterfly shuffles. Every lane then runs the
-// per-particle epilogue (speed, Euler, heading, seam wrap) and the lanes split the point
-// location candidates of the face bucket. Lane 0 writes the SoA outputs, inserts the particle
-// into next step's vertex buckets and contributes to the block's order-parameter partials.
+// k_step_tiled<LANES>: each workgroup is a tile of one replica (TileDesc); LANES lanes of a wave64
+// cooperate on one particle. They stride over the particle's row of the truncated geodesic table
+// (vertices within 2.4 sigma of its vertex, ascending in distance), visit the particles bucketed on
+// those vertices, and reduce force, alignment sum and neighbour count with butterfly shuffles.
+// Every lane then runs the per-particle epilogue (speed, Euler, heading, seam wrap) and the lanes
+// split the point-location candidates of the face bucket. Lane 0 writes the SoA outputs, inserts
+// the particle into next step's vertex buckets and contributes to the tile's order-parameter
+// partials. A single system is the case of one replica.
 #include <hip/hip_runtime.h>
+
+#include <climits>
 
 #include "launch.h"
 
@@ -25,6 +28,14 @@ __device__ __forceinline__ void bucket_insert(const BucketsOut& b, int v, int i)
     const int o = atomicAdd(b.ovf_n, 1);
     b.ovf[o] = make_int2(v, i);
   }
+}
+
+// buckets of replica r within a set
+__device__ __forceinline__ BucketsIn replica_in(const BucketSet& s, int r, int nv, int off) {
+  return {s.count + (size_t)r * nv, s.slot + (size_t)r * nv * kSlots, s.ovf_n + r, s.ovf + off};
+}
+__device__ __forceinline__ BucketsOut replica_out(const BucketSet& s, int r, int nv, int off) {
+  return {s.count + (size_t)r * nv, s.slot + (size_t)r * nv * kSlots, s.ovf_n + r, s.ovf + off};
 }
 
 // block-wide deterministic sum of (sgn, cs, sn): butterfly within each wave64, then wave
@@ -48,51 +59,67 @@ __device__ void block_order_partial(int sgn, double cs, double sn, OrderPartial*
   }
 }
 
-template <int LANES>
-__global__ void __launch_bounds__(kStepMaxBlock) k_step(MeshDev mesh, StepParams prm, int n, BucketsIn bin, BucketsOut bout,
-                                               int* clear_count, int* clear_ovf_n, StepIO io) {
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & (LANES - 1);
-  const int i = tid / LANES;
-  // a step after a validation failure passes the state through unchanged (the reference stops
-  // at the first failure)
-  const bool halted = *(volatile const unsigned*)io.flags != 0u;
-  // clear the bucket set consumed two steps ago (it is next step's output)
-  for (int v = tid; v < mesh.nv; v += gridDim.x * blockDim.x) clear_count[v] = 0;
-  if (tid == 0) *clear_ovf_n = 0;
+template <int LANES, bool TAPE>
+__global__ void __launch_bounds__(kStepMaxBlock) k_step_tiled(MeshDev mesh, const ReplicaDesc* reps,
+                                                              const TileDesc* tiles, BucketSet bin, BucketSet bout,
+                                                              BucketSet bclr, TiledIO io) {
+  const TileDesc tile = tiles[blockIdx.x];
+  if (tile.r < 0) return;  // idle padding workgroup (whole block)
+  const int r = tile.r;
+  const ReplicaDesc& rep = reps[r];  // uniform: read through the scalar cache where used
+  const int nv = mesh.nv, tid = threadIdx.x;
+  // clear this replica's bucket set consumed two steps ago (it is next step's output)
+  int* clr = bclr.count + (size_t)r * nv;
+  for (int v = tile.q * (int)blockDim.x + tid; v < nv; v += tile.nq * (int)blockDim.x) clr[v] = 0;
+  // a replica flagged before this step passes its state through unchanged (the reference stops
+  // at the first failure); the flag rows rotate like the bucket sets
+  const unsigned halt = io.halt_in[r];
+  if (tile.q == 0 && tid == 0) {
+    bclr.ovf_n[r] = 0;
+    io.halt_clr[r] = 0u;
+    if (halt) atomicOr(&io.halt_out[r], halt);
+  }
+  const BucketsOut bo = replica_out(bout, r, nv, rep.off);
+  const int lane = tid & (LANES - 1);
+  const int i = tile.first + tid / LANES;
 
   int sgn = 0;
   double pcs = 0.0, psn = 0.0;
-  if (i < n) {
-    const float4 me = io.xycs_in[i];
-    const float th = io.th_in[i];
-    const int v = io.vid_in[i];
-    if (halted) {
+  if (i < rep.n) {
+    const int g = rep.off + i;
+    const float4 me = io.xycs_in[g];
+    const float th = io.th_in[g];
+    const int v = io.vid_in[g];
+    if (halt) {
       if (lane == 0) {
-        io.xycs_out[i] = me;
-        io.th_out[i] = th;
-        io.vid_out[i] = v;
-        bucket_insert(bout, v, i);
+        io.xycs_out[g] = me;
+        io.th_out[g] = th;
+        io.vid_out[g] = v;
+        bucket_insert(bo, v, g);
       }
     } else {
-      const float eta = io.noise ? io.noise[i] : 0.0f;
-      const GlobalSource src{bin, io.xycs_in, io.th_in};
-      const StepOut o = particle_step<LANES>(mesh, prm, src, i, me, th, v, eta, lane);
+      const float eta = io.noise ? io.noise[g] : 0.0f;
+      const GlobalSource src{replica_in(bin, r, nv, rep.off), io.xycs_in, io.th_in};
+      const StepOut o = particle_step<LANES>(mesh, rep.prm, src, g, me, th, v, eta, lane);
       if (lane == 0) {
-        io.xycs_out[i] = make_float4(o.xo, o.yo, o.co, o.so);
-        io.th_out[i] = o.to;
-        io.vid_out[i] = o.L.vid;
-        io.vx[i] = o.vx;
-        io.vy[i] = o.vy;
-        io.r3x[i] = (float)o.L.X;
-        io.r3y[i] = (float)o.L.Y;
-        io.r3z[i] = (float)o.L.Z;
-        io.face[i] = o.L.face;
-        io.nbr[i] = o.nb;  // count_particle_neighbors (2DTissue.cpp:254-268)
-        bucket_insert(bout, o.L.vid, i);
+        io.xycs_out[g] = make_float4(o.xo, o.yo, o.co, o.so);
+        io.th_out[g] = o.to;
+        io.vid_out[g] = o.L.vid;
+        io.vx[g] = o.vx;
+        io.vy[g] = o.vy;
+        io.r3x[g] = (float)o.L.X;
+        io.r3y[g] = (float)o.L.Y;
+        io.r3z[g] = (float)o.L.Z;
+        io.face[g] = o.L.face;
+        io.nbr[g] = o.nb;  // count_particle_neighbors (2DTissue.cpp:254-268)
+        bucket_insert(bo, o.L.vid, g);
         if (o.fl) {
-          atomicOr(io.flags, o.fl);
-          atomicMin(io.first_bad, io.step);
+          atomicOr(&io.halt_out[r], o.fl);
+          atomicMin(&io.first_bad[r], io.step);
+        }
+        if constexpr (TAPE) {
+          io.tape_fs[g] = make_float4(o.Fx, o.Fy, o.Sx, o.Sy);
+          io.tape_q[g] = o.q;
         }
         sgn = o.sgn;
         pcs = (double)o.co;
@@ -100,7 +127,39 @@ __global__ void __launch_bounds__(kStepMaxBlock) k_step(MeshDev mesh, StepParams
       }
     }
   }
-  block_order_partial(sgn, pcs, psn, io.partial + blockIdx.x);
+  block_order_partial(sgn, pcs, psn, io.partial + tile.logical);
+}
+
+__global__ void k_phi_tiles(int R, int steps, const int* tile_begin, int n_logical, const OrderPartial* partial,
+                            const ReplicaDesc* reps, double* phi, int phi_stride, int t_base) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= R * steps) return;
+  const int r = idx / steps, s = idx % steps;
+  double cs = 0.0, sn = 0.0;
+  long sg = 0;
+  for (int b = tile_begin[r]; b < tile_begin[r + 1]; ++b) {
+    const OrderPartial p = partial[(size_t)s * n_logical + b];
+    cs += p.cs;
+    sn += p.sn;
+    sg += p.sgn;
+  }
+  const int n = reps[r].n;
+  double* out = phi + 2 * ((size_t)r * phi_stride + t_base + s);
+  out[0] = (1.0 / n) * fabs((double)sg);
+  out[1] = sqrt(cs * cs + sn * sn) / n;
+}
+
+__global__ void k_flags_begin(int R, const unsigned* halt, unsigned* rows, int* first_bad) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  rows[r] = halt[r];
+  rows[R + r] = 0u;
+  first_bad[r] = INT_MAX;
+}
+
+__global__ void k_flags_end(int R, const unsigned* last_row, unsigned* halt) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < R) halt[r] = last_row[r];
 }
 
 __global__ void k_prepare(int n, const float* x, const float* y, const float* th, float4* xycs, float* th_out) {
@@ -123,66 +182,79 @@ __global__ void k_locate(MeshDev mesh, int n, const float4* xycs, int* vid, int*
   if (r3x) { r3x[i] = (float)L.X; r3y[i] = (float)L.Y; r3z[i] = (float)L.Z; }
 }
 
-__global__ void k_clear(int nv, int* count, int* ovf_n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nv) count[i] = 0;
-  if (i == 0) *ovf_n = 0;
+__global__ void k_insert(int ntot, const int* vid, const int* rid, const ReplicaDesc* reps, int nv, BucketSet bs) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ntot) return;
+  const int r = rid[g];
+  bucket_insert(replica_out(bs, r, nv, reps[r].off), vid[g], g);
 }
 
-__global__ void k_insert(int n, const int* vid, BucketsOut bout) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) bucket_insert(bout, vid[i], i);
-}
-
-__global__ void k_order_finalize(const OrderPartial* partial, int steps, int blocks, int n, double* phi) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= steps) return;
-  double cs = 0.0, sn = 0.0;
-  long sg = 0;
-  for (int b = 0; b < blocks; ++b) {
-    const OrderPartial p = partial[(size_t)t * blocks + b];
-    cs += p.cs;
-    sn += p.sn;
-    sg += p.sgn;
-  }
-  phi[2 * t] = (1.0 / n) * fabs((double)sg);
-  phi[2 * t + 1] = sqrt(cs * cs + sn * sn) / n;
-}
-
-__global__ void k_neighbor_lists(MeshDev mesh, StepParams prm, int n, BucketsIn bin, const int* vid, int cap,
-                                 int* counts, int* cols) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int v = vid[i];
+__global__ void k_neighbor_lists(MeshDev mesh, const ReplicaDesc* reps, const int* rid, int ntot, BucketSet bs,
+                                 const int* vid, int cap, int* counts, int* cols) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ntot) return;
+  const int r = rid[g];
+  const int off = reps[r].off;
+  const float t_near = reps[r].prm.t_near;
+  const BucketsIn b = replica_in(bs, r, mesh.nv, off);
+  const int v = vid[g];
   int k = 0;
   for (int e = mesh.row_ptr[v]; e < mesh.row_ptr[v + 1]; ++e) {
     const int2 ent = mesh.nbr[e];
-    if (__int_as_float(ent.y) > prm.t_near) break;
-    for_bucket(bin, ent.x, [&](int j) {
-      if (k < cap) cols[(size_t)i * cap + k] = j;
+    if (__int_as_float(ent.y) > t_near) break;
+    for_bucket(b, ent.x, [&](int j) {
+      if (k < cap) cols[(size_t)g * cap + k] = j - off;
       ++k;
     });
   }
-  counts[i] = k;
+  counts[g] = k;
 }
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
+template <int LANES>
+void launch_tiled_lanes(int block, bool tape, const MeshDev& mesh, const ReplicaDesc* reps, const TileDesc* tiles,
+                        int n_tiles, const BucketSet& bin, const BucketSet& bout, const BucketSet& bclr,
+                        const TiledIO& io, hipStream_t st) {
+  if (tape)
+    hipLaunchKernelGGL((k_step_tiled<LANES, true>), dim3(n_tiles), dim3(block), 0, st, mesh, reps, tiles, bin, bout,
+                       bclr, io);
+  else
+    hipLaunchKernelGGL((k_step_tiled<LANES, false>), dim3(n_tiles), dim3(block), 0, st, mesh, reps, tiles, bin,
+                       bout, bclr, io);
+}
+
 }  // namespace
 
-int step_blocks(int n, int lanes, int block) { return cdiv((long)n * lanes, block); }
-
-hipError_t launch_step(int lanes, int block, const MeshDev& mesh, const StepParams& prm, int n, const BucketsIn& bin,
-                       const BucketsOut& bout, int* clear_count, int* clear_ovf_n, const StepIO& io, hipStream_t st) {
-  const int blocks = step_blocks(n, lanes, block);
+hipError_t launch_step_tiled(int lanes, int block, bool tape, const MeshDev& mesh, const ReplicaDesc* reps,
+                             const TileDesc* tiles, int n_tiles, const BucketSet& bin, const BucketSet& bout,
+                             const BucketSet& bclr, const TiledIO& io, hipStream_t st) {
+  if (block < 64 || block > kStepMaxBlock || block % 64 || n_tiles < 1) return hipErrorInvalidValue;
   switch (lanes) {
-    case 1: hipLaunchKernelGGL(k_step<1>, dim3(blocks), dim3(block), 0, st, mesh, prm, n, bin, bout, clear_count, clear_ovf_n, io); break;
-    case 2: hipLaunchKernelGGL(k_step<2>, dim3(blocks), dim3(block), 0, st, mesh, prm, n, bin, bout, clear_count, clear_ovf_n, io); break;
-    case 4: hipLaunchKernelGGL(k_step<4>, dim3(blocks), dim3(block), 0, st, mesh, prm, n, bin, bout, clear_count, clear_ovf_n, io); break;
-    case 8: hipLaunchKernelGGL(k_step<8>, dim3(blocks), dim3(block), 0, st, mesh, prm, n, bin, bout, clear_count, clear_ovf_n, io); break;
-    case 16: hipLaunchKernelGGL(k_step<16>, dim3(blocks), dim3(block), 0, st, mesh, prm, n, bin, bout, clear_count, clear_ovf_n, io); break;
+    case 1: launch_tiled_lanes<1>(block, tape, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st); break;
+    case 2: launch_tiled_lanes<2>(block, tape, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st); break;
+    case 4: launch_tiled_lanes<4>(block, tape, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st); break;
+    case 8: launch_tiled_lanes<8>(block, tape, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st); break;
+    case 16: launch_tiled_lanes<16>(block, tape, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_phi_tiles(int R, int steps, const int* tile_begin, int n_logical, const OrderPartial* partial,
+                            const ReplicaDesc* reps, double* phi, int phi_stride, int t_base, hipStream_t st) {
+  hipLaunchKernelGGL(k_phi_tiles, dim3(cdiv((long)R * steps, 64)), dim3(64), 0, st, R, steps, tile_begin, n_logical,
+                     partial, reps, phi, phi_stride, t_base);
+  return hipGetLastError();
+}
+
+hipError_t launch_flags_begin(int R, const unsigned* halt, unsigned* rows, int* first_bad, hipStream_t st) {
+  hipLaunchKernelGGL(k_flags_begin, dim3(cdiv(R, 64)), dim3(64), 0, st, R, halt, rows, first_bad);
+  return hipGetLastError();
+}
+
+hipError_t launch_flags_end(int R, const unsigned* last_row, unsigned* halt, hipStream_t st) {
+  hipLaunchKernelGGL(k_flags_end, dim3(cdiv(R, 64)), dim3(64), 0, st, R, last_row, halt);
   return hipGetLastError();
 }
 
@@ -198,25 +270,17 @@ hipError_t launch_locate(const MeshDev& mesh, int n, const float4* xycs, int* vi
   return hipGetLastError();
 }
 
-hipError_t launch_clear(int nv, int* count, int* ovf_n, hipStream_t st) {
-  hipLaunchKernelGGL(k_clear, dim3(cdiv(nv > 0 ? nv : 1, 256)), dim3(256), 0, st, nv, count, ovf_n);
+hipError_t launch_insert(int ntot, const int* vid, const int* rid, const ReplicaDesc* reps, int nv,
+                         const BucketSet& bs, hipStream_t st) {
+  hipLaunchKernelGGL(k_insert, dim3(cdiv(ntot, 256)), dim3(256), 0, st, ntot, vid, rid, reps, nv, bs);
   return hipGetLastError();
 }
 
-hipError_t launch_insert(int n, const int* vid, const BucketsOut& bout, hipStream_t st) {
-  hipLaunchKernelGGL(k_insert, dim3(cdiv(n, 256)), dim3(256), 0, st, n, vid, bout);
-  return hipGetLastError();
-}
-
-hipError_t launch_order_finalize(const OrderPartial* partial, int steps, int blocks, int n, double* phi,
+hipError_t launch_neighbor_lists(const MeshDev& mesh, const ReplicaDesc* reps, const int* rid, int ntot,
+                                 const BucketSet& bs, const int* vid, int cap, int* counts, int* cols,
                                  hipStream_t st) {
-  hipLaunchKernelGGL(k_order_finalize, dim3(cdiv(steps, 64)), dim3(64), 0, st, partial, steps, blocks, n, phi);
-  return hipGetLastError();
-}
-
-hipError_t launch_neighbor_lists(const MeshDev& mesh, const StepParams& prm, int n, const BucketsIn& bin,
-                                 const int* vid, int cap, int* counts, int* cols, hipStream_t st) {
-  hipLaunchKernelGGL(k_neighbor_lists, dim3(cdiv(n, 256)), dim3(256), 0, st, mesh, prm, n, bin, vid, cap, counts, cols);
+  hipLaunchKernelGGL(k_neighbor_lists, dim3(cdiv(ntot, 256)), dim3(256), 0, st, mesh, reps, rid, ntot, bs, vid, cap,
+                     counts, cols);
   return hipGetLastError();
 }
 

@@ -1,5 +1,5 @@
-// Host-side launchers for the kernels in kernels.hip (plain C++ signatures, callable from the
-// C-ABI layer).
+// Host-side launchers for the kernels in kernels.hip and resident.hip (plain C++ signatures,
+// callable from the C-ABI layer).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -12,8 +12,32 @@ struct OrderPartial {
   int sgn, pad;    // sum of sign(x vy - y vx)
 };
 
-// everything one step reads and writes besides the mesh and the buckets
-struct StepIO {
+// one replica: particles [off, off + n) of the concatenated arrays
+struct ReplicaDesc {
+  int off, n;
+  tdev::StepParams prm;
+};
+
+// one workgroup of the tiled step kernel: particles [first, first + block / lanes) of replica r
+struct TileDesc {
+  int r;        // replica; -1 marks an idle workgroup (padding of the XCD-balanced order)
+  int first;    // first particle of the tile, counted within the replica
+  int q, nq;    // tile index within the replica, number of tiles of the replica
+  int logical;  // slot of the tile's order-parameter partial (tiles of a replica are consecutive)
+  int pad;
+};
+
+// one vertex-bucket set of all replicas: replica r owns count[r*nv ..], slot[r*nv*kSlots ..],
+// ovf_n[r] and the overflow entries ovf[off_r .. off_r + n_r)
+struct BucketSet {
+  int* count;
+  int* slot;
+  int* ovf_n;
+  int2* ovf;
+};
+
+// everything one step of the tiled kernel reads and writes besides the mesh and the buckets
+struct TiledIO {
   const float4* xycs_in;  // (x, y, cos theta, sin theta)
   const float* th_in;
   const int* vid_in;
@@ -27,17 +51,15 @@ struct StepIO {
   float* r3z;
   int* face;
   int* nbr;
-  const float* noise;      // eta of this step, [n] (may be null)
-  OrderPartial* partial;   // [blocks] for this step
-  unsigned* flags;
-  int* first_bad;
-  int step;
-};
-
-// one replica of the persistent kernel: particles [off, off + n) of the batch arrays
-struct ReplicaDesc {
-  int off, n;
-  tdev::StepParams prm;
+  const float* noise;       // eta of this step, [ntot] (may be null: zero noise)
+  OrderPartial* partial;    // [logical tiles] of this step
+  const unsigned* halt_in;  // [R] flags raised before this step (a flagged replica passes through)
+  unsigned* halt_out;       // [R] flags raised up to and including this step
+  unsigned* halt_clr;       // [R] the third flag row, cleared for the next step
+  int* first_bad;           // [R] first step of the run that raised a flag
+  int step;                 // step index within the run
+  float4* tape_fs;          // adjoint tape of this step, [ntot] (Fx, Fy, Sx, Sy); used by the TAPE variant
+  int* tape_q;              // [ntot] quarter turns of the seam wrap
 };
 
 struct ResidentIO {
@@ -56,14 +78,17 @@ struct ResidentIO {
   int* nbr;
   const float* noise;     // eta of the launch's first step; step s at noise + s * noise_stride
   size_t noise_stride;
-  double* phi;            // (phi_ref, phi_polar) of replica r, global step t at phi[2 * (r * phi_stride + t)]
+  double* phi;            // (phi_ref, phi_polar) of replica r, run step t at phi[2 * (r * phi_stride + t)]
   int phi_stride;
-  int t_base;             // global step index of the launch's first step
-  unsigned* flags;        // [R]
-  int* first_bad;         // [R]
+  int t_base;             // run step index of the launch's first step
+  const unsigned* flags_in;  // [R] flags before the launch
+  unsigned* flags_out;       // [R] flags after the launch
+  int* first_bad;            // [R]
 };
 
-// largest workgroup of the persistent kernel (its register budget is set for 8 waves per CU)
+// largest workgroup of the tiled step kernel and of the persistent kernel (register budget of
+// two waves per SIMD)
+constexpr int kStepMaxBlock = 512;
 constexpr int kResidentMaxBlock = 512;
 
 // lanes per particle for the persistent kernel when not tuned: wider groups for smaller replicas
@@ -78,23 +103,30 @@ hipError_t resident_prepare(int nv, int nmax);
 hipError_t launch_resident(int lanes, int block, const tdev::MeshDev& mesh, const ReplicaDesc* reps, int R, int nmax,
                            int steps, const ResidentIO& io, hipStream_t st);
 
-// largest workgroup of the one-launch-per-step kernel (register budget of 2 waves per SIMD)
-constexpr int kStepMaxBlock = 512;
+// one step of every replica: n_tiles workgroups of `block` threads, `lanes` lanes per particle;
+// reads bucket set bin, writes bout, clears bclr; tape selects the variant that records the
+// adjoint tape
+hipError_t launch_step_tiled(int lanes, int block, bool tape, const tdev::MeshDev& mesh, const ReplicaDesc* reps,
+                             const TileDesc* tiles, int n_tiles, const BucketSet& bin, const BucketSet& bout,
+                             const BucketSet& bclr, const TiledIO& io, hipStream_t st);
+// order parameters of `steps` steps of every replica from the tiles' partials
+// (partial[s * n_logical + tile]); replica r's tiles are tile_begin[r] .. tile_begin[r+1]
+hipError_t launch_phi_tiles(int R, int steps, const int* tile_begin, int n_logical, const OrderPartial* partial,
+                            const ReplicaDesc* reps, double* phi, int phi_stride, int t_base, hipStream_t st);
+// run start: rows[0] = halt, rows[1] = 0, first_bad = INT_MAX; run end: halt = last row
+hipError_t launch_flags_begin(int R, const unsigned* halt, unsigned* rows, int* first_bad, hipStream_t st);
+hipError_t launch_flags_end(int R, const unsigned* last_row, unsigned* halt, hipStream_t st);
 
-int step_blocks(int n, int lanes, int block);
-hipError_t launch_step(int lanes, int block, const tdev::MeshDev& mesh, const tdev::StepParams& prm, int n,
-                       const tdev::BucketsIn& bin, const tdev::BucketsOut& bout, int* clear_count, int* clear_ovf_n,
-                       const StepIO& io, hipStream_t st);
 hipError_t launch_prepare(int n, const float* x, const float* y, const float* th, float4* xycs, float* th_out,
                           hipStream_t st);
 hipError_t launch_locate(const tdev::MeshDev& mesh, int n, const float4* xycs, int* vid, int* face, float* r3x,
                          float* r3y, float* r3z, hipStream_t st);
-hipError_t launch_clear(int nv, int* count, int* ovf_n, hipStream_t st);
-hipError_t launch_insert(int n, const int* vid, const tdev::BucketsOut& bout, hipStream_t st);
-hipError_t launch_order_finalize(const OrderPartial* partial, int steps, int blocks, int n, double* phi,
-                                 hipStream_t st);
-hipError_t launch_neighbor_lists(const tdev::MeshDev& mesh, const tdev::StepParams& prm, int n,
-                                 const tdev::BucketsIn& bin, const int* vid, int cap, int* counts, int* cols,
+// bucket every particle of every replica (rid[g] = replica of particle g) into set `bs`
+hipError_t launch_insert(int ntot, const int* vid, const int* rid, const ReplicaDesc* reps, int nv,
+                         const BucketSet& bs, hipStream_t st);
+// neighbour sets {j : d_ij < 2 sigma} of the current state (ids counted within the replica)
+hipError_t launch_neighbor_lists(const tdev::MeshDev& mesh, const ReplicaDesc* reps, const int* rid, int ntot,
+                                 const BucketSet& bs, const int* vid, int cap, int* counts, int* cols,
                                  hipStream_t st);
 
 }  // namespace tissue

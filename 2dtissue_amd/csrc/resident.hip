@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_resident(MeshDev mesh, const Repl
     th[i] = io.th[off + i];
     vid[i] = io.vid[off + i];
   }
-  if (tid == 0) s_flags = io.flags[r];
+  if (tid == 0) s_flags = io.flags_in[r];
   __syncthreads();
 
   const LdsSource src{begin, sorted, xycs, th};
@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_resident(MeshDev mesh, const Repl
     io.th[off + i] = th[i];
     io.vid[off + i] = vid[i];
   }
-  if (tid == 0) io.flags[r] = s_flags;
+  if (tid == 0) io.flags_out[r] = s_flags;
 }
 
 template <int LANES>

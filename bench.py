@@ -12,8 +12,8 @@ is `value`; the persistent LDS-resident kernel on the same system is reported be
 Also reported:
   batched: 256 replicas on the same mesh, sizes uniform in [100, 4000] (seed 3), tau log-uniform
     in [0.1, 10], noise std uniform in [0, 0.5], v0 uniform in [0.5, 2]; replica r uses init seed
-    2000 + r and noise seed 1000 + r; 500 steps in one hipGraph-captured run of the persistent
-    kernel, after the autotuner has picked its launch shape. Compared with launching the plain
+    2000 + r and noise seed 1000 + r; 500 steps in one hipGraph-captured run, after the autotuner
+    has picked the execution mode (tiled step kernel or persistent kernel) and launch shape. Compared with launching the plain
     per-step kernel once per replica (timed on 16 replicas spread over the size range).
   cpu_baseline: the FP64 oracle's dense reference algorithm on a bounded sample, single thread.
 Roofline: algorithmic bytes per particle-step (64 B: read x, y, theta, vertex id, eta; write x,
@@ -111,9 +111,11 @@ def batched(T, mesh, steps, n_replicas=256, plain_sample=16):
     b.init_random(init_seeds)
     log(f"batched: generating {steps} noise rows for {b.ntot} particles")
     b.gen_noise(noise_seeds, std, steps)
-    log("batched: autotuning the launch shape")
-    lanes, block = b.autotune(20)
-    log(f"batched: lanes {lanes}, block {block}; timing {steps} steps")
+    log("batched: autotuning the execution mode and launch shape")
+    b.autotune(20)
+    shape = b.launch()
+    lanes, block = shape["lanes"], shape["block"]
+    log(f"batched: {shape}; timing {steps} steps")
     b.run(steps, 0)  # warm the graph (captured and instantiated on first use)
     b.init_random(init_seeds)
     t0 = time.perf_counter()
@@ -125,8 +127,10 @@ def batched(T, mesh, steps, n_replicas=256, plain_sample=16):
     out = dict(workload=f"{n_replicas} replicas, sizes U[100,4000] (total {b.ntot}), {steps} steps",
                value=ps_batched, unit="particle-steps/s", total_particles=int(b.ntot), steps=steps,
                elapsed_s=elapsed, graph_ms=run_ms, status=code, replicas_flagged=int((flags != 0).sum()),
-               lanes=lanes, block=block,
-               roofline=roofline(b.ntot, run_ms / 1e3 / steps, "k_resident (one step, all replicas)"))
+               launch=shape,
+               roofline=roofline(b.ntot, run_ms / 1e3 / steps,
+                                 "k_step_tiled (one step, all replicas)" if shape["mode"] == 1
+                                 else "k_resident (one step, all replicas)"))
     b.close()
     # the same replicas, each launched alone through the plain per-step kernel
     pick = np.argsort(sizes)[np.linspace(0, n_replicas - 1, plain_sample).astype(int)]

@@ -111,21 +111,30 @@ int32_t tissue_set_launch(tissue_sim* s, int32_t lanes, int32_t block);
 /* execution mode of tissue_step*: one kernel launch per step over the whole GPU, or the
  * persistent LDS-resident kernel (one workgroup, many steps per launch; needs
  * n <= tissue_resident_capacity). AUTO picks the resident kernel only for n <= 256 (one pass of
- * a workgroup), the step kernel otherwise. */
+ * a workgroup), the step kernel otherwise, unless tissue_autotune chose for this size. */
 #define TISSUE_MODE_AUTO 0
 #define TISSUE_MODE_STEP_KERNEL 1
 #define TISSUE_MODE_RESIDENT 2
 int32_t tissue_set_mode(tissue_sim* s, int32_t mode);
 int32_t tissue_resident_capacity(const tissue_mesh* m, int32_t* max_particles);
+/* the launch shape the next step uses, shape[5] = {mode, lanes, block, steps per launch,
+ * xcd order (1: each replica's tiles on one XCD)} */
+int32_t tissue_get_launch(tissue_sim* s, int32_t* shape);
+/* time n_steps of every mode and launch shape that fits (state restored after each trial), keep
+ * the fastest for AUTO; shape[5] as tissue_get_launch (may be NULL) */
+int32_t tissue_autotune(tissue_sim* s, int32_t n_steps, int32_t* shape);
 int32_t tissue_synchronize(tissue_sim* s);
 /* GPU time of the step kernels of the last tissue_step call (HIP events on the handle's stream) */
 int32_t tissue_last_step_ms(tissue_sim* s, double* ms);
 
-/* ---- many independent replicas in one persistent launch ----
+/* ---- many independent replicas, one hipGraph per run ----
  * Replica r owns particles [off_r, off_r + n[r]) of the concatenated arrays (off_r = sum of the
- * earlier n). Noise rows are [steps][sum n] with replica r's columns at off_r. One workgroup runs
- * one replica with its state resident in LDS; the launches of a run are captured in a hipGraph
- * and replayed. A replica that raises a validation flag stops (its later steps are no-ops). */
+ * earlier n). Noise rows are [steps][sum n] with replica r's columns at off_r. Two modes: the
+ * tiled step kernel (one launch per step covers every replica, workgroups are tiles of one
+ * replica, optionally placed so that each replica stays on one XCD) or the persistent kernel (one
+ * workgroup per replica, state resident in LDS, many steps per launch). The launches of a run are
+ * captured in a hipGraph and replayed. A replica that raises a validation flag stops (its later
+ * steps are no-ops, also in later runs, until its state is set again). */
 typedef struct tissue_batch tissue_batch;
 int32_t tissue_batch_create(const tissue_mesh* m, int32_t n_replicas, const int32_t* n, const tissue_params* params,
                             tissue_batch** out);
@@ -142,10 +151,17 @@ int32_t tissue_batch_run(tissue_batch* b, int32_t n_steps, int32_t first_row);
 int32_t tissue_batch_get_order(tissue_batch* b, int32_t replica, double* phi_ref, double* phi_polar, int32_t cap,
                                int32_t* count);
 int32_t tissue_batch_get_flags(tissue_batch* b, uint32_t* flags /* [n_replicas] */, int32_t* first_step);
-/* lanes per particle (1, 2, 4, 8), threads per workgroup (64..512), steps per launch; 0 = auto */
+/* lanes per particle (1, 2, 4, 8, 16; the persistent kernel uses at most 8), threads per
+ * workgroup (64..512), steps per launch; 0 = auto */
 int32_t tissue_batch_set_launch(tissue_batch* b, int32_t lanes, int32_t block, int32_t steps_per_launch);
-/* time n_steps of every launch shape that fits (state restored after each trial) and keep the
- * fastest; results are cached per (max replica size, mesh, sigma) for the process */
+/* TISSUE_MODE_* and the tiled mode's workgroup order: xcd_order 1 keeps each replica on one XCD,
+ * 0 is replica order, -1 automatic */
+int32_t tissue_batch_set_mode(tissue_batch* b, int32_t mode, int32_t xcd_order);
+/* shape[5] = {mode, lanes, block, steps per launch, xcd order} of the next run */
+int32_t tissue_batch_get_launch(tissue_batch* b, int32_t* shape);
+/* time n_steps of every mode and launch shape that fits (state restored after each trial) and
+ * keep the fastest for AUTO; results are cached per (replicas, largest replica, particles, mesh)
+ * for the process */
 int32_t tissue_batch_autotune(tissue_batch* b, int32_t n_steps, int32_t* lanes, int32_t* block);
 /* GPU time of the last tissue_batch_run (HIP events around the graph launch) */
 int32_t tissue_batch_last_run_ms(tissue_batch* b, double* ms);

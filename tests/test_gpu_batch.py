@@ -1,13 +1,17 @@
-"""GPU tests of the persistent LDS-resident kernel: single simulations (resident mode) and
-batches of replicas. Both kernels share the per-particle update and add neighbours in the same
-order, so with the same lanes per particle their results are bitwise equal; the batch must
-reproduce each replica run alone."""
+"""GPU tests of the two execution modes and of batches of replicas.
+
+Mode 1 is the tiled step kernel (one launch per step over every replica), mode 2 the persistent
+LDS-resident kernel. Both call the same per-particle update and add neighbours in the same
+(ascending id) order, so with the same lanes per particle their states are bitwise equal; a
+batch must reproduce each replica run alone, in either mode and either workgroup order."""
 import numpy as np
 import pytest
 
 import parity as P
 
 pytestmark = pytest.mark.gpu
+
+TILED, RESIDENT = 1, 2
 
 
 def make_sim(T, m, cfg, mode, lanes, block=0):
@@ -19,18 +23,20 @@ def make_sim(T, m, cfg, mode, lanes, block=0):
     return s
 
 
+def same_state(a, b):
+    return all(np.array_equal(getattr(a, k), getattr(b, k)) for k in ("x", "y", "theta", "vid"))
+
+
 @pytest.mark.parametrize("lanes", [1, 2, 8])
 def test_resident_equals_step_kernel(T, lanes):
     cfg = P.C2
     m, _ = P.mesh_and_tables(cfg["freq"], cfg["sigma"])
     noise = T.gen_noise(cfg["noise_seed"], 0.1, 30, cfg["n"])
-    a = make_sim(T, m, cfg, T.MODE_STEP_KERNEL, lanes)
-    b = make_sim(T, m, cfg, T.MODE_RESIDENT, lanes)
+    a = make_sim(T, m, cfg, TILED, lanes)
+    b = make_sim(T, m, cfg, RESIDENT, lanes)
     assert a.step(30, noise) == T.OK
     assert b.step(30, noise) == T.OK
-    sa, sb = a.get_state(), b.get_state()
-    for k in ("x", "y", "theta", "vid"):
-        assert np.array_equal(getattr(sa, k), getattr(sb, k)), k
+    assert same_state(a.get_state(), b.get_state())
     oa, ob = a.get_outputs(), b.get_outputs()
     for k in ("vx", "vy", "r3d", "face", "nbr"):
         assert np.array_equal(getattr(oa, k), getattr(ob, k)), k
@@ -44,7 +50,7 @@ def test_resident_teacher_forced_c1(T, O):
     m, tb = P.mesh_and_tables(cfg["freq"], cfg["sigma"])
     om = O.Mesh(tb)
     p = dict(P.PHYS, sigma=cfg["sigma"])
-    sim = make_sim(T, m, cfg, T.MODE_RESIDENT, 0)
+    sim = make_sim(T, m, cfg, RESIDENT, 0)
     noise = T.gen_noise(cfg["noise_seed"], 0.1, 50, cfg["n"])
     total = {}
     for t in range(50):
@@ -58,12 +64,30 @@ def test_resident_teacher_forced_c1(T, O):
     assert all(v == 0 for v in total.values()), total
 
 
-BATCH_SIZES = [100, 517, 1000, 2000, 3100, 4000]
+@pytest.mark.parametrize("mode", [TILED, RESIDENT])
+def test_split_runs_equal_one_run(T, mode):
+    """Runs are replayed hipGraphs: 3 runs of 7 steps (the first graph replayed) and one run
+    of 21 steps from the same state give bitwise the same state and order history."""
+    cfg = P.C2
+    m, _ = P.mesh_and_tables(cfg["freq"], cfg["sigma"])
+    noise = T.gen_noise(cfg["noise_seed"], 0.1, 21, cfg["n"])
+    a = make_sim(T, m, cfg, mode, 2)
+    b = make_sim(T, m, cfg, mode, 2)
+    a.stage_noise(noise)
+    b.stage_noise(noise)
+    for k in range(3):
+        assert a.step_staged(7, 7 * k) == T.OK
+    assert b.step_staged(21, 0) == T.OK
+    assert same_state(a.get_state(), b.get_state())
+    assert np.array_equal(a.order()[0], b.order()[0])
+    assert np.allclose(a.order()[1], b.order()[1], rtol=0, atol=1e-12)
 
 
-def batch_params(T, sigma):
+BATCH_SIZES = [100, 517, 1000, 2000, 3100, 4000, 250, 1500, 777]
+
+
+def batch_params(T, sigma, R):
     rng = np.random.default_rng(3)
-    R = len(BATCH_SIZES)
     tau = np.exp(rng.uniform(np.log(0.1), np.log(10.0), R))
     std = rng.uniform(0.0, 0.5, R)
     v0 = rng.uniform(0.5, 2.0, R)
@@ -71,14 +95,17 @@ def batch_params(T, sigma):
     return ps, std
 
 
-def test_batch_equals_solo_runs(T):
+@pytest.mark.parametrize("mode,xcd", [(TILED, 0), (TILED, 1), (RESIDENT, 0)])
+def test_batch_equals_solo_runs(T, mode, xcd):
     sigma = 0.05
     m, _ = P.mesh_and_tables(32, sigma)
-    ps, std = batch_params(T, sigma)
+    ps, std = batch_params(T, sigma, len(BATCH_SIZES))
     seeds = np.array([1000 + r for r in range(len(BATCH_SIZES))], np.uint64)
     steps = 40
+    lanes = 1 if mode == RESIDENT else 2
     b = T.Batch(m, BATCH_SIZES, ps)
-    b.set_launch(1, 512, 16)  # several launches per run
+    b.set_mode(mode, xcd)
+    b.set_launch(lanes, 512, 16)  # several launches / reductions per run
     b.init_random(seeds)
     b.gen_noise(seeds, std, steps)
     assert b.run(steps, 0) == T.OK
@@ -87,8 +114,8 @@ def test_batch_equals_solo_runs(T):
     for r, n in enumerate(BATCH_SIZES):
         o = int(b.off[r])
         s = T.Sim(m, ps[r], n)
-        s.set_mode(T.MODE_RESIDENT)
-        s.set_launch(1, 512)
+        s.set_mode(mode)
+        s.set_launch(lanes, 512)
         s.init_random(int(seeds[r]))
         assert s.step(steps, T.gen_noise(int(seeds[r]), float(std[r]), steps, n)) == T.OK
         ss = s.get_state()
@@ -105,24 +132,26 @@ def test_batch_equals_solo_runs(T):
 def test_batch_autotune_restores_state(T):
     sigma = 0.05
     m, _ = P.mesh_and_tables(32, sigma)
-    ps, std = batch_params(T, sigma)
+    ps, std = batch_params(T, sigma, len(BATCH_SIZES))
     seeds = np.arange(len(BATCH_SIZES), dtype=np.uint64) + 7
     b = T.Batch(m, BATCH_SIZES, ps)
     b.init_random(seeds)
     b.gen_noise(seeds, std, 10)
     before = b.get_state()
     lanes, block = b.autotune(10)
-    assert lanes in (1, 2, 4, 8) and block in (256, 512)
-    after = b.get_state()
-    assert all(np.array_equal(getattr(before, k), getattr(after, k)) for k in ("x", "y", "theta", "vid"))
+    assert lanes in (1, 2, 4, 8) and block in (128, 256, 512)
+    assert b.launch()["mode"] in (TILED, RESIDENT)
+    assert same_state(before, b.get_state())
     assert len(b.order(0)[0]) == 0
     assert b.run(10, 0) == T.OK
     assert len(b.order(0)[0]) == 10
 
 
-def test_batch_failure_is_per_replica(T):
+@pytest.mark.parametrize("mode", [TILED, RESIDENT])
+def test_batch_failure_is_per_replica(T, mode):
     """A replica whose seam wrap exceeds its crossing bound (forced from the host with
-    max_crossings = 1) stops and reports; the other replicas run on exactly as alone."""
+    max_crossings = 1) stops and reports; the others run on exactly as alone. The stopped
+    replica stays stopped in later runs."""
     sigma = 0.05
     m, _ = P.mesh_and_tables(32, sigma)
     sizes = [800, 800, 800]
@@ -130,24 +159,31 @@ def test_batch_failure_is_per_replica(T):
     ps[1] = T.params(sigma=sigma, v0=1.0, k=10.0, mu=1.0, tau=1.0, dt=0.02, max_crossings=1)
     seeds = np.array([11, 12, 13], np.uint64)
     b = T.Batch(m, sizes, ps)
+    b.set_mode(mode, 0)
     b.set_launch(2, 512, 0)
     b.init_random(seeds)
-    b.gen_noise(seeds, np.array([0.1, 0.1, 0.1]), 30)
+    b.gen_noise(seeds, np.array([0.1, 0.1, 0.1]), 40)
     code = b.run(30, 0)
     flags, first = b.flags()
     assert code == T.E_TILING_NO_CONVERGE
     assert flags[1] & T.FLAG_TILING and flags[0] == 0 and flags[2] == 0
     assert 0 <= first[1] < 30
     s = T.Sim(m, ps[2], 800)
-    s.set_mode(T.MODE_RESIDENT)
+    s.set_mode(mode)
     s.set_launch(2, 512)
     s.init_random(13)
     assert s.step(30, T.gen_noise(13, 0.1, 30, 800)) == T.OK
     st = b.get_state()
     assert np.array_equal(s.get_state().x, st.x[1600:2400])
+    # a later run leaves the stopped replica as it was and raises nothing new
+    assert b.run(10, 30) == T.OK
+    st2 = b.get_state()
+    assert np.array_equal(st.x[800:1600], st2.x[800:1600])
+    assert b.flags()[0][1] == 0
+    assert not np.array_equal(st.x[1600:2400], st2.x[1600:2400])
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [TILED, RESIDENT])
 def test_sim_nonfinite_noise_reported_and_halts(T, mode):
     cfg = P.C1
     m, _ = P.mesh_and_tables(cfg["freq"], cfg["sigma"])
@@ -162,7 +198,7 @@ def test_sim_nonfinite_noise_reported_and_halts(T, mode):
     assert len(r) == 5 and r[3] == 0.0 and r[4] == 0.0
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [TILED, RESIDENT])
 def test_sim_crossing_bound_reported(T, mode):
     cfg = P.C2
     m, _ = P.mesh_and_tables(cfg["freq"], cfg["sigma"])
