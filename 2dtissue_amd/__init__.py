@@ -51,6 +51,53 @@ def params(sigma=0.05, v0=1.0, k=10.0, mu=1.0, tau=1.0, dt=0.02, max_crossings=0
     return Params(sigma, v0, k, mu, tau, dt, max_crossings, 0)
 
 
+MAX_MODES = 16
+
+
+class Loss(C.Structure):
+    """tissue_loss: w_order (phi_T - target_order)^2 + w_velocity mean |rdot - u(r)|^2, with
+    u_c(x, y) = sum of amp sin(kx x + ky y + phase) over the modes of component c."""
+    _fields_ = [("target_order", C.c_double), ("w_order", C.c_double), ("w_velocity", C.c_double),
+                ("n_modes", C.c_int32), ("reserved", C.c_int32), ("component", C.c_int32 * MAX_MODES),
+                ("amp", C.c_double * MAX_MODES), ("kx", C.c_double * MAX_MODES), ("ky", C.c_double * MAX_MODES),
+                ("phase", C.c_double * MAX_MODES)]
+
+
+class Grads(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("loss", "loss_order", "loss_velocity", "phi_final", "d_tau", "d_mu", "d_v0",
+                                          "d_k", "forward_ms", "reverse_ms")]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def loss(target_order=0.8, w_order=1.0, w_velocity=0.1, modes=()) -> Loss:
+    """modes: (component, amp, kx, ky, phase) tuples of the target field u."""
+    if len(modes) > MAX_MODES:
+        raise ValueError(f"at most {MAX_MODES} modes")
+    L = Loss()
+    L.target_order, L.w_order, L.w_velocity, L.n_modes = target_order, w_order, w_velocity, len(modes)
+    for m, (c, a, kx, ky, ph) in enumerate(modes):
+        L.component[m], L.amp[m], L.kx[m], L.ky[m], L.phase[m] = int(c), a, kx, ky, ph
+    return L
+
+
+def target_field(seed: int = 4, per_component: int = 4) -> list:
+    """A seeded smooth velocity field on the UV square: per component, modes with integer
+    wave numbers in {-2..2} (times 2 pi, not both zero), amplitudes U[-1, 1], phases U[0, 2 pi)."""
+    rng = np.random.default_rng(seed)
+    modes = []
+    for c in (0, 1):
+        for _ in range(per_component):
+            while True:
+                kx, ky = rng.integers(-2, 3, 2)
+                if kx or ky:
+                    break
+            modes.append((c, float(rng.uniform(-1, 1)), 2 * np.pi * float(kx), 2 * np.pi * float(ky),
+                          float(rng.uniform(0, 2 * np.pi))))
+    return modes
+
+
 _lib = None
 
 # (name, restype, argtypes) of every entry point declared in include/tissue.h
@@ -105,6 +152,10 @@ SIGNATURES = [
     ("tissue_batch_get_launch", C.c_int32, [_P, _I]),
     ("tissue_batch_autotune", C.c_int32, [_P, C.c_int32, _I, _I]),
     ("tissue_batch_last_run_ms", C.c_int32, [_P, _D]),
+    ("tissue_adjoint", C.c_int32, [_P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(Loss), C.POINTER(Grads), _D, _D,
+                                   _D]),
+    ("tissue_batch_adjoint", C.c_int32, [_P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(Loss), C.POINTER(Grads), _D,
+                                         _D, _D]),
     ("tissue_gen_noise", C.c_int32, [C.c_uint64, C.c_double, C.c_int32, C.c_int32, _F]),
     ("tissue_philox4x32", None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
 ]
@@ -324,6 +375,17 @@ class Sim:
         _check(lib().tissue_autotune(self.h, n_steps, _ptr(s, C.c_int32)))
         return _shape_dict(s)
 
+    def adjoint(self, n_steps: int, loss: Loss, first_row: int = -1, ckpt_every: int = 20):
+        """Forward n_steps (state advances) and reverse; returns (status, gradients dict,
+        d_theta0, d_x0, d_y0)."""
+        g = Grads()
+        dth = np.zeros(self.n); dx = np.zeros(self.n); dy = np.zeros(self.n)
+        code = lib().tissue_adjoint(self.h, n_steps, first_row, ckpt_every, C.byref(loss), C.byref(g),
+                                    _ptr(dth, C.c_double), _ptr(dx, C.c_double), _ptr(dy, C.c_double))
+        if code not in (OK, E_LOST_PARTICLE, E_NONFINITE, E_TILING_NO_CONVERGE):
+            _check(code)
+        return code, g.as_dict(), dth, dx, dy
+
     def synchronize(self) -> None:
         _check(lib().tissue_synchronize(self.h))
 
@@ -436,6 +498,17 @@ class Batch:
         s = np.zeros(5, np.int32)
         _check(lib().tissue_batch_get_launch(self.h, _ptr(s, C.c_int32)))
         return _shape_dict(s)
+
+    def adjoint(self, n_steps: int, loss: Loss, first_row: int = -1, ckpt_every: int = 20):
+        """Every replica forward and reverse together; returns (status, [gradients dict per
+        replica], d_theta0, d_x0, d_y0) with the d_* arrays over all particles."""
+        g = (Grads * self.R)()
+        dth = np.zeros(self.ntot); dx = np.zeros(self.ntot); dy = np.zeros(self.ntot)
+        code = lib().tissue_batch_adjoint(self.h, n_steps, first_row, ckpt_every, C.byref(loss), g,
+                                          _ptr(dth, C.c_double), _ptr(dx, C.c_double), _ptr(dy, C.c_double))
+        if code not in (OK, E_LOST_PARTICLE, E_NONFINITE, E_TILING_NO_CONVERGE):
+            _check(code)
+        return code, [x.as_dict() for x in g], dth, dx, dy
 
     def autotune(self, n_steps: int) -> tuple[int, int]:
         l = C.c_int32(); b = C.c_int32()

@@ -1,0 +1,293 @@
+// Reverse pass of the step (gfx950), through the forward's frozen decisions.
+//
+// Step t maps the state (r_i, theta_i) to
+//   p_i      = r_i + dt s_i e_i,  e_i = (cos theta_i, sin theta_i),  s_i = v0 + mu |F_i|,
+//   F_i      = sum_{j in A_i, j != i} c_ij (r_i - r_j),    c_ij = f(d~_ij) / d~_ij  (frozen d_ij),
+//   r_i'     = R_i p_i + b_i                                (seam wrap: rotation by q_i quarter turns),
+//   theta_i' = theta_i + rho wrap(atan2(S_i, C_i) - theta_i) + eta_i + const,  rho = dt / tau,
+//   S_i, C_i = sum_{j in A_i} (sin, cos) theta_j            (A_i includes i),
+// and adds w_v |rdot_i - u(r_i)|^2 to the loss (w_v = w_velocity / (T n)).
+// Its adjoint, for incoming (lr_i, lt_i) of the new state:
+//   lp = R_i^T lr;  lrdot = dt lp + 2 w_v (rdot - u);  ls = lrdot . e;  lF = mu ls F / |F|;
+//   lr_i  = lp - 2 w_v J_u^T (rdot - u) + sum_{j in A_i, j != i} c_ij (lF_i - lF_j);
+//   lS_i  = rho lt_i C_i / Q_i,  lC_i = -rho lt_i S_i / Q_i,  Q_i = C_i^2 + S_i^2;
+//   lt_i  = (1 - rho) lt_i + s_i lrdot . (-sin, cos) + cos theta_i sum_{j in A_i} lS_j - sin theta_i sum_{j in A_i} lC_j.
+// The neighbour relation is symmetric (the geodesic table is min-symmetrised), so the transposed
+// sums are gathers over the particle's own neighbour set: k_adj_local computes the per-particle
+// factors, k_adj_gather walks the same geodesic rows and vertex buckets as the forward step.
+// The adjoint state of replica r is held as lambda * 2^-E_r (FP32, E_r >= 0 in sexp[r]): the map
+// is linear in lambda, so only the loss sources are scaled by 2^-E_r and the FP64 parameter
+// accumulators by 2^E_r; k_adj_rescale raises E_r whenever |lambda| outgrows 2^20.
+#include <hip/hip_runtime.h>
+
+#include "adjoint.h"
+
+using namespace tdev;
+
+namespace tissue {
+
+namespace {
+
+__device__ __forceinline__ void field_eval(const LossDev& L, float x, float y, float& ux, float& uy, float& jxx,
+                                           float& jxy, float& jyx, float& jyy) {
+  ux = uy = jxx = jxy = jyx = jyy = 0.f;
+#pragma unroll
+  for (int m = 0; m < kMaxModes; ++m) {
+    if (m < L.n_modes) {
+      float s, c;
+      sincosf(L.kx[m] * x + L.ky[m] * y + L.phase[m], &s, &c);
+      const float v = L.amp[m] * s, d = L.amp[m] * c;
+      if (L.comp[m] == 0) { ux += v; jxx += d * L.kx[m]; jxy += d * L.ky[m]; }
+      else { uy += v; jyx += d * L.kx[m]; jyy += d * L.ky[m]; }
+    }
+  }
+}
+
+// R_q^T v for a rotation by q counter-clockwise quarter turns
+__device__ __forceinline__ float2 rot_t(int q, float2 v) {
+  switch (q & 3) {
+    case 1: return make_float2(v.y, -v.x);
+    case 2: return make_float2(-v.x, -v.y);
+    case 3: return make_float2(-v.y, v.x);
+    default: return v;
+  }
+}
+
+// deterministic FP64 sum over a block (wave butterflies, waves added in order)
+__device__ double block_sum(double v, double* tmp) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) tmp[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int w = 0; w < nw; ++w) s += tmp[w];
+  return s;
+}
+
+// one block per replica: polar order of the final state and the seeds of the heading adjoint
+__global__ void k_adj_seed(const ReplicaDesc* reps, const float4* xycs, LossDev L, const unsigned* halt, float2* lr,
+                           float* lt, double* out) {
+  __shared__ double tmp[kAdjBlock / 64];
+  const int r = blockIdx.x;
+  const int off = reps[r].off, n = reps[r].n;
+  double cx = 0.0, cy = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const float4 q = xycs[off + i];
+    cx += (double)q.z;
+    cy += (double)q.w;
+  }
+  const double Px = block_sum(cx, tmp);
+  const double Py = block_sum(cy, tmp);
+  const double norm = sqrt(Px * Px + Py * Py);
+  const double phi = norm / n;
+  const bool ok = halt[r] == 0u;
+  // d phi / d theta_i = (-Px sin theta_i + Py cos theta_i) / (n |P|)
+  const double coef = (ok && norm > 0.0) ? L.w_order * 2.0 * (phi - L.target) / (n * norm) : 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const float4 q = xycs[off + i];
+    lr[off + i] = make_float2(0.f, 0.f);
+    lt[off + i] = (float)(coef * (-Px * (double)q.w + Py * (double)q.z));
+  }
+  if (threadIdx.x == 0) {
+    out[2 * r] = phi;
+    out[2 * r + 1] = L.w_order * (phi - L.target) * (phi - L.target);
+  }
+}
+
+__global__ void k_adj_local(const ReplicaDesc* reps, const int* rid, int ntot, LossDev L, const unsigned* halt,
+                            const int* sexp, const float4* xycs, const float* th, const float4* fs, const int* quarter,
+                            const float2* lr_in, const float* lt_in, float2* gF, float2* gSC, float4* part,
+                            double* acc) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ntot) return;
+  const int r = rid[g];
+  if (halt[r]) {
+    gF[g] = make_float2(0.f, 0.f);
+    gSC[g] = make_float2(0.f, 0.f);
+    part[g] = make_float4(0.f, 0.f, 0.f, 0.f);
+    return;
+  }
+  const StepParams& prm = reps[r].prm;
+  const int E = sexp[r];
+  const float src = ldexpf(1.f, -E);        // loss sources in the scaled units
+  const double unscale = ldexp(1.0, E);     // back to true units for the accumulators
+  const float4 me = xycs[g];
+  const float theta = th[g];
+  const float4 f = fs[g];  // (Fx, Fy, sum cos, sum sin)
+  const float2 lp = rot_t(quarter[g], lr_in[g]);
+  const float ltn = lt_in[g];
+  const float m = sqrtf(f.x * f.x + f.y * f.y);
+  const float s = prm.v0 + prm.mu * m;
+  const float rdx = me.z * s, rdy = me.w * s;
+  float ux, uy, jxx, jxy, jyx, jyy;
+  field_eval(L, me.x, me.y, ux, uy, jxx, jxy, jyx, jyy);
+  const float wv = L.w_velocity / ((float)L.n_steps * (float)reps[r].n);
+  const float dvx = rdx - ux, dvy = rdy - uy;
+  const float sw = 2.f * wv * src;
+  const float lrdx = prm.dt * lp.x + sw * dvx, lrdy = prm.dt * lp.y + sw * dvy;
+  const float lrx = lp.x - sw * (jxx * dvx + jyx * dvy);
+  const float lry = lp.y - sw * (jxy * dvx + jyy * dvy);
+  const float ls = lrdx * me.z + lrdy * me.w;
+  const float lm = prm.mu * ls;
+  const float2 lF = m > 0.f ? make_float2(lm * f.x / m, lm * f.y / m) : make_float2(0.f, 0.f);
+  const float Q = f.z * f.z + f.w * f.w;
+  const float lmean = prm.rate * ltn;
+  const float2 lsc = Q > 0.f ? make_float2(lmean * f.z / Q, -lmean * f.w / Q) : make_float2(0.f, 0.f);
+  const float lth = (1.f - prm.rate) * ltn + s * (-lrdx * me.w + lrdy * me.z);
+  gF[g] = lF;
+  gSC[g] = lsc;  // (d loss / d sum sin, d loss / d sum cos)
+  part[g] = make_float4(lrx, lry, lth, 0.f);
+  // parameter gradients and the velocity loss, accumulated per particle in FP64
+  double delta = atan2((double)f.w, (double)f.z) - (double)theta;
+  if (delta > kPi) delta -= 2.0 * kPi;
+  else if (delta <= -kPi) delta += 2.0 * kPi;
+  const double tau = prm.dt64 / prm.rate64;
+  acc[g] += unscale * (double)ls;                                            // d v0
+  acc[(size_t)ntot + g] += unscale * (double)ls * (double)m;                 // d mu
+  if (prm.k64 != 0.0)                                                        // d k
+    acc[2 * (size_t)ntot + g] += unscale * ((double)lF.x * f.x + (double)lF.y * f.y) / prm.k64;
+  acc[3 * (size_t)ntot + g] += unscale * (double)ltn * delta * (-prm.dt64 / (tau * tau));  // d tau
+  acc[4 * (size_t)ntot + g] += (double)wv * ((double)dvx * dvx + (double)dvy * dvy);       // loss
+}
+
+template <int LANES>
+__global__ void __launch_bounds__(256) k_adj_gather(MeshDev mesh, const ReplicaDesc* reps, const int* rid, int ntot,
+                                                   BucketSet bs, const unsigned* halt, const float4* xycs,
+                                                   const int* vid, const float2* gF, const float2* gSC,
+                                                   const float4* part, float2* lr_out, float* lt_out) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = tid / LANES, lane = tid & (LANES - 1);
+  if (g >= ntot) return;  // whole groups (ntot * LANES threads are grouped by LANES)
+  const int r = rid[g];
+  if (halt[r]) {
+    if (lane == 0) { lr_out[g] = make_float2(0.f, 0.f); lt_out[g] = 0.f; }
+    return;
+  }
+  const ReplicaDesc& rep = reps[r];
+  const StepParams& prm = rep.prm;
+  const int nv = mesh.nv;
+  const BucketsIn b{bs.count + (size_t)r * nv, bs.slot + (size_t)r * nv * kSlots, bs.ovf_n + r, bs.ovf + rep.off};
+  const float2 mine = gF[g];
+  float ax = 0.f, ay = 0.f, aS = 0.f, aC = 0.f;
+  const int v = vid[g];
+  const int rs = mesh.row_ptr[v], re = mesh.row_ptr[v + 1];
+  for (int e = rs + lane; e < re; e += LANES) {
+    const int2 ent = mesh.nbr[e];
+    const float d = __int_as_float(ent.y);
+    if (d > prm.t_near) break;  // rows ascend in distance
+    const float c = pair_coef(d, prm);
+    for_bucket(b, ent.x, [&](int j) {
+      const float2 oj = gF[j];
+      const float2 sc = gSC[j];
+      if (j != g) {
+        ax += c * (mine.x - oj.x);
+        ay += c * (mine.y - oj.y);
+      }
+      aS += sc.x;
+      aC += sc.y;
+    });
+  }
+  ax = group_sum<LANES>(ax);
+  ay = group_sum<LANES>(ay);
+  aS = group_sum<LANES>(aS);
+  aC = group_sum<LANES>(aC);
+  if (lane == 0) {
+    const float4 pt = part[g];
+    const float4 me = xycs[g];
+    lr_out[g] = make_float2(pt.x + ax, pt.y + ay);
+    lt_out[g] = pt.z + me.z * aS - me.w * aC;
+  }
+}
+
+// one block per replica: keep |lambda| below 2^21 by exact power-of-two rescaling
+__global__ void k_adj_rescale(const ReplicaDesc* reps, float2* lr, float* lt, int* sexp) {
+  __shared__ float tmp[kAdjBlock / 64];
+  const int r = blockIdx.x;
+  const int off = reps[r].off, n = reps[r].n;
+  float mx = 0.f;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const float2 a = lr[off + i];
+    mx = fmaxf(mx, fmaxf(fabsf(a.x), fmaxf(fabsf(a.y), fabsf(lt[off + i]))));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+  if ((threadIdx.x & 63) == 0) tmp[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) mx = fmaxf(mx, tmp[w]);
+  if (!(mx > 0.f) || !isfinite(mx)) return;  // uniform over the block
+  const int s = ilogbf(mx) - 20;
+  if (s <= 0) return;
+  const float f = ldexpf(1.f, -s);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const float2 a = lr[off + i];
+    lr[off + i] = make_float2(a.x * f, a.y * f);
+    lt[off + i] *= f;
+  }
+  if (threadIdx.x == 0) sexp[r] += s;
+}
+
+// one block per replica: sums of the per-particle accumulators
+__global__ void k_adj_reduce(const ReplicaDesc* reps, int ntot, const double* acc, double* out) {
+  __shared__ double tmp[kAdjBlock / 64];
+  const int r = blockIdx.x;
+  const int off = reps[r].off, n = reps[r].n;
+  for (int k = 0; k < kAdjAcc; ++k) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += acc[(size_t)k * ntot + off + i];
+    s = block_sum(s, tmp);
+    if (threadIdx.x == 0) out[kAdjAcc * r + k] = s;
+  }
+}
+
+inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+}  // namespace
+
+hipError_t launch_adj_seed(int R, const ReplicaDesc* reps, const float4* xycs, const LossDev& L, const unsigned* halt,
+                           float2* lr, float* lt, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_seed, dim3(R), dim3(kAdjBlock), 0, st, reps, xycs, L, halt, lr, lt, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_local(const ReplicaDesc* reps, const int* rid, int ntot, const LossDev& L, const unsigned* halt,
+                            const int* sexp, const float4* xycs, const float* th, const float4* fs, const int* quarter,
+                            const float2* lr_in, const float* lt_in, float2* gF, float2* gSC, float4* part,
+                            double* acc, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_local, dim3(cdiv(ntot, 256)), dim3(256), 0, st, reps, rid, ntot, L, halt, sexp, xycs, th,
+                     fs, quarter, lr_in, lt_in, gF, gSC, part, acc);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_rescale(int R, const ReplicaDesc* reps, float2* lr, float* lt, int* sexp, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_rescale, dim3(R), dim3(kAdjBlock), 0, st, reps, lr, lt, sexp);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_gather(int lanes, const MeshDev& mesh, const ReplicaDesc* reps, const int* rid, int ntot,
+                             const BucketSet& bs, const unsigned* halt, const float4* xycs, const int* vid,
+                             const float2* gF, const float2* gSC, const float4* part, float2* lr_out, float* lt_out,
+                             hipStream_t st) {
+  const int blocks = cdiv((long)ntot * lanes, 256);
+#define TISSUE_ADJ_GATHER(L)                                                                                     \
+  hipLaunchKernelGGL(k_adj_gather<L>, dim3(blocks), dim3(256), 0, st, mesh, reps, rid, ntot, bs, halt, xycs, vid, \
+                     gF, gSC, part, lr_out, lt_out)
+  switch (lanes) {
+    case 1: TISSUE_ADJ_GATHER(1); break;
+    case 2: TISSUE_ADJ_GATHER(2); break;
+    case 4: TISSUE_ADJ_GATHER(4); break;
+    case 8: TISSUE_ADJ_GATHER(8); break;
+    case 16: TISSUE_ADJ_GATHER(16); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef TISSUE_ADJ_GATHER
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_reduce(int R, const ReplicaDesc* reps, int ntot, const double* acc, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_adj_reduce, dim3(R), dim3(kAdjBlock), 0, st, reps, ntot, acc, out);
+  return hipGetLastError();
+}
+
+}  // namespace tissue

@@ -7,10 +7,6 @@
 
 #include "engine.h"
 
-struct tissue_sim {
-  std::unique_ptr<tissue::Engine> e;
-};
-
 namespace {
 
 tissue::Engine& eng(tissue_sim* s) {

@@ -112,6 +112,7 @@ void Engine::release() {
   dfree(d_halt); dfree(d_rows); dfree(d_first_bad);
   for (int k = 0; k < 3; ++k) { dfree(bs[k].count); dfree(bs[k].slot); dfree(bs[k].ovf_n); dfree(bs[k].ovf); }
   dfree(d_tiles); dfree(d_tile_begin); dfree(partial); dfree(phi); dfree(noise); dfree(call_noise);
+  adjoint_release();
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   if (stream) (void)hipStreamDestroy(stream);
@@ -334,6 +335,21 @@ Shape Engine::resolve() const {
     s.mode = (R == 1 && ntot <= kAutoResidentMax && nmax <= resident_cap) ? TISSUE_MODE_RESIDENT
                                                                            : TISSUE_MODE_STEP_KERNEL;
   }
+  return resolve_for(s);
+}
+
+Shape Engine::resolve_tiled() const {
+  Shape s = req;
+  if (s.mode == TISSUE_MODE_AUTO && s.lanes == 0 && s.block == 0) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find({R, nmax, ntot, md.nv});
+    if (it != g_tuned.end() && it->second.mode == TISSUE_MODE_STEP_KERNEL) return it->second;
+  }
+  s.mode = TISSUE_MODE_STEP_KERNEL;
+  return resolve_for(s);
+}
+
+Shape Engine::resolve_for(Shape s) const {
   if (s.mode == TISSUE_MODE_RESIDENT) {
     if (nmax > resident_cap) throw ApiError(TISSUE_E_INVALID_ARG, "replica too large for the resident kernel on this mesh");
     s.lanes = s.lanes > 0 ? std::min(s.lanes, 8) : resident_default_lanes(nmax);
@@ -399,11 +415,12 @@ void Engine::enqueue(const Shape& s, int n_steps, const float* nz, size_t stride
   HIP_OK(launch_flags_end(R, d_rows + (size_t)(n_steps % 3) * R, d_halt, stream));
 }
 
-int32_t Engine::run(int n_steps, const float* nz, size_t stride) {
+int32_t Engine::run(int n_steps, const float* nz, size_t stride) { return run_shape(resolve(), n_steps, nz, stride); }
+
+int32_t Engine::run_shape(const Shape& s, int n_steps, const float* nz, size_t stride) {
   if (!has_state) throw ApiError(TISSUE_E_INVALID_ARG, "no state: call init_random or set_state first");
   if (n_steps < 0) throw ApiError(TISSUE_E_INVALID_ARG, "negative step count");
   if (n_steps == 0) return TISSUE_OK;
-  const Shape s = resolve();
   const size_t need_phi = 2 * (size_t)R * n_steps;
   if (phi_cap < need_phi) {
     HIP_OK(hipStreamSynchronize(stream));

@@ -5,9 +5,11 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <memory>
 #include <tuple>
 #include <vector>
 
+#include "adjoint.h"
 #include "capi_common.h"
 
 namespace tissue {
@@ -45,10 +47,17 @@ struct Engine {
 
   // n_steps steps; row s of the noise at noise + s * stride (null: no noise). Returns the status.
   int32_t run(int n_steps, const float* noise, size_t stride);
+  int32_t run_shape(const Shape& s, int n_steps, const float* noise, size_t stride);
   Shape resolve() const;
+  // the request resolved for the tiled step kernel (the adjoint records its tape there)
+  Shape resolve_tiled() const;
   Shape autotune(int n_steps);
   void set_request(const Shape& s);
   void synchronize();
+  // gradients of a trajectory loss over n_steps from the current state (engine_adjoint.cpp);
+  // g[R]; d_theta0, d_x0, d_y0 [ntot] may be null
+  int32_t adjoint(int n_steps, const float* noise, size_t stride, int ckpt_every, const tissue_loss& loss,
+                  tissue_grads* g, double* d_theta0, double* d_x0, double* d_y0);
 
   Shape req;
   std::vector<std::vector<double>> hist_ref, hist_polar;
@@ -100,12 +109,40 @@ struct Engine {
 
   std::map<std::tuple<int, const float*, size_t, std::tuple<int, int, int, int, int>, int, int>, hipGraphExec_t> graphs;
 
+  // adjoint buffers (grown on demand): checkpoints [C][ntot], the segment tape [K+1][ntot] of
+  // states and [K][ntot] of force / alignment sums and seam-wrap turns, adjoint state, factors
+  struct AdjointBuffers {
+    int K = 0, C = 0;
+    float4* ck_xycs = nullptr; float* ck_th = nullptr; int* ck_vid = nullptr;
+    float4* tp_xycs = nullptr; float* tp_th = nullptr; int* tp_vid = nullptr;
+    float4* tp_fs = nullptr; int* tp_q = nullptr;
+    float2* lr[2] = {nullptr, nullptr}; float* lt[2] = {nullptr, nullptr};
+    float2* gF = nullptr; float2* gSC = nullptr; float4* part = nullptr;
+    double* acc = nullptr; double* out_seed = nullptr; double* out_red = nullptr;
+    float *vx = nullptr, *vy = nullptr, *r3x = nullptr, *r3y = nullptr, *r3z = nullptr;
+    int *face = nullptr, *nbr = nullptr;
+    unsigned* rows = nullptr; int* first_bad = nullptr;
+    int* sexp = nullptr;  // per replica: the adjoint state is stored scaled by 2^-sexp
+    OrderPartial* partial = nullptr; int partial_n = 0;
+  } adj;
+  void adjoint_buffers(int K, int C);
+  void adjoint_release();
+
   void drop_graphs();
   void release();
   void ensure_buckets();
   void rebuild_buckets();
   void ensure_tiles(const Shape& s);
   void enqueue(const Shape& s, int n_steps, const float* noise, size_t stride);
+  Shape resolve_for(Shape s) const;
 };
 
 }  // namespace tissue
+
+// the C-ABI handles: a single simulation is the one-replica case of the engine
+struct tissue_sim {
+  std::unique_ptr<tissue::Engine> e;
+};
+struct tissue_batch {
+  std::unique_ptr<tissue::Engine> e;
+};

@@ -1,0 +1,244 @@
+// Adjoint of n_steps of the step (engine side) and its C-ABI entry points.
+//
+// Forward: the tiled step kernel, with the state copied to a checkpoint every ckpt_every steps.
+// Reverse, segment by segment from the last: the segment is recomputed from its checkpoint by
+// the recording variant of the same kernel (bitwise the forward's states), which stores every
+// pre-step state, the force and alignment sums and the seam-wrap turns; then, step by step
+// backwards, the step's vertex buckets are rebuilt from its recorded vertex ids and the two
+// reverse kernels (per-particle factors, then the transposed neighbour gather) map the adjoint
+// of the new state to the adjoint of the old one.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cmath>
+
+#include "engine.h"
+
+namespace tissue {
+
+namespace {
+
+LossDev make_loss(const tissue_loss& l, int n_steps) {
+  if (l.n_modes < 0 || l.n_modes > kMaxModes) throw ApiError(TISSUE_E_INVALID_ARG, "n_modes out of range");
+  const double sc[] = {l.target_order, l.w_order, l.w_velocity};
+  for (double x : sc)
+    if (!std::isfinite(x)) throw ApiError(TISSUE_E_INVALID_ARG, "non-finite loss weight");
+  LossDev d{};
+  d.n_modes = l.n_modes;
+  d.n_steps = n_steps;
+  d.target = (float)l.target_order;
+  d.w_order = (float)l.w_order;
+  d.w_velocity = (float)l.w_velocity;
+  for (int m = 0; m < l.n_modes; ++m) {
+    if (l.component[m] != 0 && l.component[m] != 1) throw ApiError(TISSUE_E_INVALID_ARG, "mode component must be 0 or 1");
+    const double v[] = {l.amp[m], l.kx[m], l.ky[m], l.phase[m]};
+    for (double x : v)
+      if (!std::isfinite(x)) throw ApiError(TISSUE_E_INVALID_ARG, "non-finite field mode");
+    d.comp[m] = l.component[m];
+    d.amp[m] = (float)l.amp[m];
+    d.kx[m] = (float)l.kx[m];
+    d.ky[m] = (float)l.ky[m];
+    d.phase[m] = (float)l.phase[m];
+  }
+  return d;
+}
+
+}  // namespace
+
+void Engine::adjoint_release() {
+  AdjointBuffers& a = adj;
+  dfree(a.ck_xycs); dfree(a.ck_th); dfree(a.ck_vid);
+  dfree(a.tp_xycs); dfree(a.tp_th); dfree(a.tp_vid); dfree(a.tp_fs); dfree(a.tp_q);
+  for (int k = 0; k < 2; ++k) { dfree(a.lr[k]); dfree(a.lt[k]); }
+  dfree(a.gF); dfree(a.gSC); dfree(a.part); dfree(a.acc); dfree(a.out_seed); dfree(a.out_red);
+  dfree(a.vx); dfree(a.vy); dfree(a.r3x); dfree(a.r3y); dfree(a.r3z); dfree(a.face); dfree(a.nbr);
+  dfree(a.rows); dfree(a.first_bad); dfree(a.partial); dfree(a.sexp);
+  a = AdjointBuffers{};
+}
+
+void Engine::adjoint_buffers(int K, int C) {
+  AdjointBuffers& a = adj;
+  const size_t N = ntot;
+  if (!a.lr[0]) {
+    for (int k = 0; k < 2; ++k) { a.lr[k] = dalloc<float2>(N); a.lt[k] = dalloc<float>(N); }
+    a.gF = dalloc<float2>(N); a.gSC = dalloc<float2>(N); a.part = dalloc<float4>(N);
+    a.acc = dalloc<double>(kAdjAcc * N);
+    a.out_seed = dalloc<double>(2 * (size_t)R);
+    a.out_red = dalloc<double>(kAdjAcc * (size_t)R);
+    a.vx = dalloc<float>(N); a.vy = dalloc<float>(N);
+    a.r3x = dalloc<float>(N); a.r3y = dalloc<float>(N); a.r3z = dalloc<float>(N);
+    a.face = dalloc<int>(N); a.nbr = dalloc<int>(N);
+    a.rows = dalloc<unsigned>(3 * (size_t)R);
+    a.first_bad = dalloc<int>(R);
+    a.sexp = dalloc<int>(R);
+  }
+  if (C > a.C) {
+    dfree(a.ck_xycs); dfree(a.ck_th); dfree(a.ck_vid);
+    a.ck_xycs = dalloc<float4>((size_t)C * N); a.ck_th = dalloc<float>((size_t)C * N); a.ck_vid = dalloc<int>((size_t)C * N);
+    a.C = C;
+  }
+  if (K > a.K) {
+    dfree(a.tp_xycs); dfree(a.tp_th); dfree(a.tp_vid); dfree(a.tp_fs); dfree(a.tp_q);
+    a.tp_xycs = dalloc<float4>((size_t)(K + 1) * N);
+    a.tp_th = dalloc<float>((size_t)(K + 1) * N);
+    a.tp_vid = dalloc<int>((size_t)(K + 1) * N);
+    a.tp_fs = dalloc<float4>((size_t)K * N);
+    a.tp_q = dalloc<int>((size_t)K * N);
+    a.K = K;
+  }
+  if (a.partial_n < n_logical) {
+    dfree(a.partial);
+    a.partial = dalloc<OrderPartial>(n_logical);
+    a.partial_n = n_logical;
+  }
+}
+
+int32_t Engine::adjoint(int T, const float* nz, size_t stride, int K, const tissue_loss& loss, tissue_grads* g,
+                        double* d_theta0, double* d_x0, double* d_y0) {
+  if (!has_state) throw ApiError(TISSUE_E_INVALID_ARG, "no state: call init_random or set_state first");
+  if (T < 1 || K < 1) throw ApiError(TISSUE_E_INVALID_ARG, "n_steps and ckpt_every must be positive");
+  if (!g) throw ApiError(TISSUE_E_INVALID_ARG, "null gradient output");
+  const LossDev L = make_loss(loss, T);
+  K = std::min(K, T);
+  const int C = (T + K - 1) / K;
+  const Shape s = resolve_tiled();
+  ensure_buckets();
+  ensure_tiles(s);
+  adjoint_buffers(K, C);
+  AdjointBuffers& a = adj;
+  const size_t N = ntot;
+
+  // forward, with a checkpoint at the start of every segment
+  double fwd_ms = 0.0;
+  int32_t status = TISSUE_OK;
+  for (int c = 0; c < C; ++c) {
+    const int t0 = c * K, kc = std::min(K, T - t0);
+    HIP_OK(hipMemcpyAsync(a.ck_xycs + c * N, xycs[sphase], N * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+    HIP_OK(hipMemcpyAsync(a.ck_th + c * N, th[sphase], N * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    HIP_OK(hipMemcpyAsync(a.ck_vid + c * N, vid[sphase], N * sizeof(int), hipMemcpyDeviceToDevice, stream));
+    const int32_t st = run_shape(s, kc, nz ? nz + (size_t)t0 * stride : nullptr, stride);
+    fwd_ms += last_ms;
+    if (st != TISSUE_OK && status == TISSUE_OK) status = st;
+  }
+
+  // reverse
+  HIP_OK(hipEventRecord(ev0, stream));
+  HIP_OK(launch_adj_seed(R, d_reps, xycs[sphase], L, d_halt, a.lr[0], a.lt[0], a.out_seed, stream));
+  HIP_OK(hipMemsetAsync(a.acc, 0, kAdjAcc * N * sizeof(double), stream));
+  HIP_OK(hipMemsetAsync(a.sexp, 0, R * sizeof(int), stream));
+  int cur = 0;
+  const size_t cells = (size_t)R * md.nv;
+  auto bucket_state = [&](const int* v, int set) {
+    HIP_OK(hipMemsetAsync(bs[set].count, 0, cells * sizeof(int), stream));
+    HIP_OK(hipMemsetAsync(bs[set].ovf_n, 0, R * sizeof(int), stream));
+    HIP_OK(launch_insert(ntot, v, d_rid, d_reps, md.nv, bs[set], stream));
+  };
+  for (int c = C - 1; c >= 0; --c) {
+    const int t0 = c * K, kc = std::min(K, T - t0);
+    // recompute the segment from its checkpoint, recording the tape
+    HIP_OK(hipMemcpyAsync(a.tp_xycs, a.ck_xycs + c * N, N * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+    HIP_OK(hipMemcpyAsync(a.tp_th, a.ck_th + c * N, N * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    HIP_OK(hipMemcpyAsync(a.tp_vid, a.ck_vid + c * N, N * sizeof(int), hipMemcpyDeviceToDevice, stream));
+    for (int k = 0; k < 3; ++k) {
+      HIP_OK(hipMemsetAsync(bs[k].count, 0, cells * sizeof(int), stream));
+      HIP_OK(hipMemsetAsync(bs[k].ovf_n, 0, R * sizeof(int), stream));
+    }
+    HIP_OK(launch_insert(ntot, a.tp_vid, d_rid, d_reps, md.nv, bs[0], stream));
+    HIP_OK(launch_flags_begin(R, d_halt, a.rows, a.first_bad, stream));
+    for (int q = 0; q < kc; ++q) {
+      TiledIO io{};
+      io.xycs_in = a.tp_xycs + q * N; io.th_in = a.tp_th + q * N; io.vid_in = a.tp_vid + q * N;
+      io.xycs_out = a.tp_xycs + (q + 1) * N; io.th_out = a.tp_th + (q + 1) * N; io.vid_out = a.tp_vid + (q + 1) * N;
+      io.vx = a.vx; io.vy = a.vy; io.r3x = a.r3x; io.r3y = a.r3y; io.r3z = a.r3z; io.face = a.face; io.nbr = a.nbr;
+      io.noise = nz ? nz + (size_t)(t0 + q) * stride : nullptr;
+      io.partial = a.partial;
+      io.halt_in = a.rows + (size_t)(q % 3) * R;
+      io.halt_out = a.rows + (size_t)((q + 1) % 3) * R;
+      io.halt_clr = a.rows + (size_t)((q + 2) % 3) * R;
+      io.first_bad = a.first_bad;
+      io.step = q;
+      io.tape_fs = a.tp_fs + q * N;
+      io.tape_q = a.tp_q + q * N;
+      HIP_OK(launch_step_tiled(s.lanes, s.block, true, md, d_reps, d_tiles, n_phys, bs[q % 3], bs[(q + 1) % 3],
+                               bs[(q + 2) % 3], io, stream));
+    }
+    // backwards through the segment
+    for (int q = kc - 1; q >= 0; --q) {
+      bucket_state(a.tp_vid + q * N, 0);
+      HIP_OK(launch_adj_local(d_reps, d_rid, ntot, L, d_halt, a.sexp, a.tp_xycs + q * N, a.tp_th + q * N, a.tp_fs + q * N,
+                              a.tp_q + q * N, a.lr[cur], a.lt[cur], a.gF, a.gSC, a.part, a.acc, stream));
+      HIP_OK(launch_adj_gather(s.lanes, md, d_reps, d_rid, ntot, bs[0], d_halt, a.tp_xycs + q * N, a.tp_vid + q * N,
+                               a.gF, a.gSC, a.part, a.lr[1 - cur], a.lt[1 - cur], stream));
+      cur = 1 - cur;
+      HIP_OK(launch_adj_rescale(R, d_reps, a.lr[cur], a.lt[cur], a.sexp, stream));
+    }
+  }
+  HIP_OK(launch_adj_reduce(R, d_reps, ntot, a.acc, a.out_red, stream));
+  HIP_OK(hipEventRecord(ev1, stream));
+  std::vector<double> seed(2 * (size_t)R), red(kAdjAcc * (size_t)R);
+  std::vector<float2> lr(N);
+  std::vector<float> lt(N);
+  std::vector<int> sexp(R);
+  HIP_OK(hipMemcpyAsync(seed.data(), a.out_seed, seed.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+  HIP_OK(hipMemcpyAsync(red.data(), a.out_red, red.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+  HIP_OK(hipMemcpyAsync(lr.data(), a.lr[cur], N * sizeof(float2), hipMemcpyDeviceToHost, stream));
+  HIP_OK(hipMemcpyAsync(lt.data(), a.lt[cur], N * sizeof(float), hipMemcpyDeviceToHost, stream));
+  HIP_OK(hipMemcpyAsync(sexp.data(), a.sexp, R * sizeof(int), hipMemcpyDeviceToHost, stream));
+  HIP_OK(hipStreamSynchronize(stream));
+  float rev = 0.f;
+  HIP_OK(hipEventElapsedTime(&rev, ev0, ev1));
+  buckets_stale = true;  // the reverse pass used the bucket sets
+  for (int r = 0; r < R; ++r) {
+    tissue_grads& o = g[r];
+    o.phi_final = seed[2 * r];
+    o.loss_order = seed[2 * r + 1];
+    o.loss_velocity = red[kAdjAcc * r + 4];
+    o.loss = o.loss_order + o.loss_velocity;
+    o.d_v0 = red[kAdjAcc * r + 0];
+    o.d_mu = red[kAdjAcc * r + 1];
+    o.d_k = red[kAdjAcc * r + 2];
+    o.d_tau = red[kAdjAcc * r + 3];
+    o.forward_ms = fwd_ms;
+    o.reverse_ms = rev;
+  }
+  for (int r = 0; r < R; ++r) {
+    const double sc = std::ldexp(1.0, sexp[r]);
+    for (int i = off[r]; i < off[r] + n[r]; ++i) {
+      if (d_theta0) d_theta0[i] = sc * (double)lt[i];
+      if (d_x0) d_x0[i] = sc * (double)lr[i].x;
+      if (d_y0) d_y0[i] = sc * (double)lr[i].y;
+    }
+  }
+  last_ms = fwd_ms + rev;
+  return status;
+}
+
+}  // namespace tissue
+
+extern "C" {
+
+int32_t tissue_adjoint(tissue_sim* s, int32_t n_steps, int32_t first_row, int32_t ckpt_every, const tissue_loss* loss,
+                       tissue_grads* g, double* d_theta0, double* d_x0, double* d_y0) {
+  return guarded([&] {
+    if (!s || !s->e) throw ApiError(TISSUE_E_INVALID_ARG, "null simulation handle");
+    if (!loss) throw ApiError(TISSUE_E_INVALID_ARG, "null loss");
+    auto& e = *s->e;
+    HIP_OK(hipSetDevice(e.device));
+    const float* nz = first_row >= 0 ? e.staged_rows(first_row, n_steps) : nullptr;
+    return e.adjoint(n_steps, nz, (size_t)e.ntot, ckpt_every, *loss, g, d_theta0, d_x0, d_y0);
+  });
+}
+
+int32_t tissue_batch_adjoint(tissue_batch* b, int32_t n_steps, int32_t first_row, int32_t ckpt_every,
+                             const tissue_loss* loss, tissue_grads* g, double* d_theta0, double* d_x0, double* d_y0) {
+  return guarded([&] {
+    if (!b || !b->e) throw ApiError(TISSUE_E_INVALID_ARG, "null batch handle");
+    if (!loss) throw ApiError(TISSUE_E_INVALID_ARG, "null loss");
+    auto& e = *b->e;
+    HIP_OK(hipSetDevice(e.device));
+    const float* nz = first_row >= 0 ? e.staged_rows(first_row, n_steps) : nullptr;
+    return e.adjoint(n_steps, nz, (size_t)e.ntot, ckpt_every, *loss, g, d_theta0, d_x0, d_y0);
+  });
+}
+
+}  // extern "C"

@@ -166,6 +166,40 @@ int32_t tissue_batch_autotune(tissue_batch* b, int32_t n_steps, int32_t* lanes, 
 /* GPU time of the last tissue_batch_run (HIP events around the graph launch) */
 int32_t tissue_batch_last_run_ms(tissue_batch* b, double* ms);
 
+/* ---- adjoint: gradients of a trajectory loss (no counterpart in the reference, which has no
+ * differentiation; it differentiates n_steps of tissue_step) ----
+ * loss = w_order (phi_T - target_order)^2 + w_velocity mean_{t<T, i} |rdot_i^t - u(r_i^t)|^2
+ * phi_T = |sum_i (cos, sin)(theta_i^T)| / n (polar order of the final state); rdot_i^t and r_i^t
+ * are step t's velocity and pre-step position; u is a smooth field on the UV square, a sum of
+ * modes u_c(x, y) += amp sin(kx x + ky y + phase) with c = component (0: x, 1: y).
+ * The reverse pass runs through the forward's frozen decisions (neighbour sets, seam-wrap maps,
+ * vertex ids), recomputing each segment of ckpt_every steps from a checkpoint. */
+#define TISSUE_MAX_MODES 16
+typedef struct {
+  double target_order;
+  double w_order;
+  double w_velocity;
+  int32_t n_modes;
+  int32_t reserved;
+  int32_t component[TISSUE_MAX_MODES];
+  double amp[TISSUE_MAX_MODES], kx[TISSUE_MAX_MODES], ky[TISSUE_MAX_MODES], phase[TISSUE_MAX_MODES];
+} tissue_loss;
+typedef struct {
+  double loss, loss_order, loss_velocity, phi_final;
+  double d_tau, d_mu, d_v0, d_k;  /* dloss / d parameter */
+  double forward_ms, reverse_ms;  /* GPU time of the checkpointed forward and of the reverse pass */
+} tissue_grads;
+/* runs n_steps from the current state (noise from staged row first_row, < 0: none), leaves the
+ * final state, and returns the gradients; d_theta0 / d_x0 / d_y0 [n] (may be NULL) are dloss with
+ * respect to the initial headings and positions. The adjoint state is FP32, rescaled per replica
+ * by powers of two after every step, so gradients far beyond the FP32 range (the d == 0 repulsion
+ * rule makes them grow geometrically with the horizon) come back as FP64. */
+int32_t tissue_adjoint(tissue_sim* s, int32_t n_steps, int32_t first_row, int32_t ckpt_every, const tissue_loss* loss,
+                       tissue_grads* g, double* d_theta0, double* d_x0, double* d_y0);
+/* every replica at once, each with its own loss and gradients: g[n_replicas], d_*[sum n] */
+int32_t tissue_batch_adjoint(tissue_batch* b, int32_t n_steps, int32_t first_row, int32_t ckpt_every,
+                             const tissue_loss* loss, tissue_grads* g, double* d_theta0, double* d_x0, double* d_y0);
+
 /* ---- host helpers (no GPU) ---- */
 /* eta[t][i] = std * N(0,1) from Philox4x32-10 (Salmon et al., SC'11) + Box-Muller */
 int32_t tissue_gen_noise(uint64_t seed, double std, int32_t n_steps, int32_t n, float* out);

@@ -93,6 +93,24 @@ int32_t or_step_int_degrees(const or_mesh* m, const or_params* p, int32_t n, dou
 int32_t or_neighbor_sets(const or_mesh* m, const int32_t* vid, int32_t n, double sigma, int32_t* row_ptr /* n+1 */,
                          int32_t* cols, int32_t cap);
 
+/* ---- reverse pass (oracle/adjoint.cpp) ----
+ * loss = w_order (phi_T - target_order)^2 + w_velocity mean_{t<T, i} |rdot_i^t - u(r_i^t)|^2,
+ * u_c(x, y) = sum over modes with component c of amp sin(kx x + ky y + phase) */
+typedef struct {
+  double target_order, w_order, w_velocity;
+  int32_t n_modes;
+  const int32_t* component;
+  const double *amp, *kx, *ky, *phase;
+} or_loss;
+/* u (2) and its Jacobian (du_x/dx, du_x/dy, du_y/dx, du_y/dy) at (x, y) */
+double or_field_eval(const or_loss* L, double x, double y, double* u, double* jac);
+/* gradients along a trajectory: xs, ys, ths, vids are states 0..T, each [T+1][n]; decisions of
+ * step t come from state t. out[8] = loss, order loss, velocity loss, phi_T, d tau, d mu, d v0,
+ * d k; d_theta0, d_x0, d_y0 [n] (may be NULL) */
+int32_t or_adjoint_reverse(const or_mesh* m, const or_params* p, int32_t n, int32_t T, const double* xs,
+                           const double* ys, const double* ths, const int32_t* vids, const or_loss* L, double* out,
+                           double* d_theta0, double* d_x0, double* d_y0);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,6 +46,7 @@ def lib() -> C.CDLL:
         L.or_point_triangle_distance.restype = C.c_double
         L.or_order_parameter.restype = C.c_double
         L.or_dist_lookup.restype = C.c_double
+        L.or_field_eval.restype = C.c_double
         L.or_repulsive_adhesion_motion.argtypes = [C.c_double] * 7 + [_D]
         L.or_repulsive_adhesion_motion.restype = None
         L.or_count_particle_neighbors.argtypes = [_D, C.c_int32, C.c_double, _I]
@@ -137,3 +138,92 @@ def neighbor_sets(mesh: Mesh, vid, sigma):
         if st == OK:
             return [cols[rp[i]:rp[i + 1]].copy() for i in range(n)]
         cap *= 4
+
+
+# ---- reverse pass (oracle/adjoint.cpp) ----
+
+class OrLoss(C.Structure):
+    _fields_ = [("target_order", C.c_double), ("w_order", C.c_double), ("w_velocity", C.c_double),
+                ("n_modes", C.c_int32), ("component", C.c_void_p), ("amp", C.c_void_p), ("kx", C.c_void_p),
+                ("ky", C.c_void_p), ("phase", C.c_void_p)]
+
+
+class Loss:
+    """Trajectory loss w_order (phi_T - target)^2 + w_velocity mean |rdot - u(r)|^2; u is a sum of
+    modes u_c(x, y) += amp sin(kx x + ky y + phase), c = component (0: x, 1: y)."""
+
+    def __init__(self, target_order=0.8, w_order=1.0, w_velocity=0.1, component=(), amp=(), kx=(), ky=(),
+                 phase=()):
+        self.target_order, self.w_order, self.w_velocity = target_order, w_order, w_velocity
+        self.component = np.ascontiguousarray(component, np.int32)
+        self.amp = np.ascontiguousarray(amp, np.float64)
+        self.kx = np.ascontiguousarray(kx, np.float64)
+        self.ky = np.ascontiguousarray(ky, np.float64)
+        self.phase = np.ascontiguousarray(phase, np.float64)
+        self.s = OrLoss(target_order, w_order, w_velocity, len(self.amp), self.component.ctypes.data,
+                        self.amp.ctypes.data, self.kx.ctypes.data, self.ky.ctypes.data, self.phase.ctypes.data)
+
+    def field(self, x, y):
+        """u(x, y) for arrays x, y (numpy restatement of the same sum)."""
+        x = np.asarray(x, np.float64); y = np.asarray(y, np.float64)
+        u = [np.zeros_like(x), np.zeros_like(x)]
+        for c, a, kx, ky, ph in zip(self.component, self.amp, self.kx, self.ky, self.phase):
+            u[c] = u[c] + a * np.sin(kx * x + ky * y + ph)
+        return u[0], u[1]
+
+
+def field_eval(loss: Loss, x: float, y: float):
+    u = np.zeros(2); jac = np.zeros(4)
+    lib().or_field_eval(C.byref(loss.s), C.c_double(x), C.c_double(y), dp(u), dp(jac))
+    return u, jac.reshape(2, 2)
+
+
+def trajectory(mesh: Mesh, p: dict, x, y, theta, vid, noise):
+    """FP64 forward of len(noise) steps; returns (status, states dict of [T+1][n] arrays, velocities
+    vx, vy [T][n] of each step)."""
+    T, n = np.shape(noise)
+    xs = np.zeros((T + 1, n)); ys = np.zeros((T + 1, n)); ths = np.zeros((T + 1, n))
+    vids = np.zeros((T + 1, n), np.int32)
+    vx = np.zeros((T, n)); vy = np.zeros((T, n))
+    s = dict(x=np.array(x, np.float64), y=np.array(y, np.float64), theta=np.array(theta, np.float64),
+             vid=np.array(vid, np.int32))
+    status = OK
+    for t in range(T):
+        xs[t], ys[t], ths[t], vids[t] = s["x"], s["y"], s["theta"], s["vid"]
+        st, s, out = step(mesh, p, s["x"], s["y"], s["theta"], s["vid"], noise[t])
+        vx[t], vy[t] = out["vx"], out["vy"]
+        if st != OK and status == OK:
+            status = st
+    xs[T], ys[T], ths[T], vids[T] = s["x"], s["y"], s["theta"], s["vid"]
+    return status, dict(x=xs, y=ys, theta=ths, vid=vids), (vx, vy)
+
+
+def trajectory_loss(states: dict, vel, loss: Loss) -> dict:
+    """The loss of a trajectory from its states and the velocities of its steps (numpy)."""
+    th = states["theta"][-1]
+    n = th.shape[0]
+    phi = np.hypot(np.cos(th).sum(), np.sin(th).sum()) / n
+    lo = loss.w_order * (phi - loss.target_order) ** 2
+    vx, vy = vel
+    T = vx.shape[0]
+    ux, uy = loss.field(states["x"][:T], states["y"][:T])
+    lv = loss.w_velocity * np.mean((vx - ux) ** 2 + (vy - uy) ** 2)
+    return dict(loss=lo + lv, loss_order=lo, loss_velocity=lv, phi_final=phi)
+
+
+def adjoint_reverse(mesh: Mesh, p: dict, states: dict, loss: Loss) -> dict:
+    """FP64 gradients along the given trajectory (decisions of step t from state t)."""
+    xs = np.ascontiguousarray(states["x"], np.float64); ys = np.ascontiguousarray(states["y"], np.float64)
+    ths = np.ascontiguousarray(states["theta"], np.float64); vids = np.ascontiguousarray(states["vid"], np.int32)
+    T = xs.shape[0] - 1
+    n = xs.shape[1]
+    prm = OrParams(p["sigma"], p["v0"], p["k"], p["mu"], p["tau"], p["dt"], p.get("max_crossings", 0) or 16384)
+    out = np.zeros(8); dth = np.zeros(n); dx = np.zeros(n); dy = np.zeros(n)
+    st = lib().or_adjoint_reverse(mesh.ref, C.byref(prm), n, T, dp(xs), dp(ys), dp(ths), ip(vids), C.byref(loss.s),
+                                  dp(out), dp(dth), dp(dx), dp(dy))
+    if st != OK:
+        raise RuntimeError(f"or_adjoint_reverse: status {st}")
+    keys = ("loss", "loss_order", "loss_velocity", "phi_final", "d_tau", "d_mu", "d_v0", "d_k")
+    r = {k: float(v) for k, v in zip(keys, out)}
+    r.update(d_theta0=dth, d_x0=dx, d_y0=dy)
+    return r
