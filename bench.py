@@ -13,8 +13,13 @@ Also reported:
   batched: 256 replicas on the same mesh, sizes uniform in [100, 4000] (seed 3), tau log-uniform
     in [0.1, 10], noise std uniform in [0, 0.5], v0 uniform in [0.5, 2]; replica r uses init seed
     2000 + r and noise seed 1000 + r; 500 steps in one hipGraph-captured run, after the autotuner
-    has picked the execution mode (tiled step kernel or persistent kernel) and launch shape. Compared with launching the plain
-    per-step kernel once per replica (timed on 16 replicas spread over the size range).
+    has picked the execution mode (tiled step kernel or persistent kernel) and launch shape.
+    Compared with launching the plain per-step kernel once per replica (timed on 16 replicas
+    spread over the size range).
+  adjoint: forward + adjoint of 1,000 particles over 200 steps (checkpoints every 20), loss
+    (phi_T - 0.8)^2 + 0.1 mean |rdot - u|^2, particle-steps/s of forward and reverse together.
+  large: perturbed sphere (F = 200,000), 100,000 particles with sigma 0.01 for 500 steps, and
+    64 x 2,000 particles forward + adjoint together over 200 steps.
   cpu_baseline: the FP64 oracle's dense reference algorithm on a bounded sample, single thread.
 Roofline: algorithmic bytes per particle-step (64 B: read x, y, theta, vertex id, eta; write x,
 y, theta, vx, vy, r3d, vertex id, face, neighbour count) over the kernel's HIP-event time, against
@@ -114,7 +119,6 @@ def batched(T, mesh, steps, n_replicas=256, plain_sample=16):
     log("batched: autotuning the execution mode and launch shape")
     b.autotune(20)
     shape = b.launch()
-    lanes, block = shape["lanes"], shape["block"]
     log(f"batched: {shape}; timing {steps} steps")
     b.run(steps, 0)  # warm the graph (captured and instantiated on first use)
     b.init_random(init_seeds)
@@ -154,6 +158,99 @@ def batched(T, mesh, steps, n_replicas=256, plain_sample=16):
     return out
 
 
+def adjoint(T, mesh, n=1000, steps=200, ckpt=20):
+    """Forward + adjoint: 1,000 particles, 200 steps, noise seed 2, checkpoints every 20 steps,
+    loss (phi_T - 0.8)^2 + 0.1 mean |rdot - u|^2 with the seeded field u (seed 4); gradients with
+    respect to tau, mu, v0, k and the initial headings and positions."""
+    w = WORKLOAD
+    p = T.params(sigma=w["sigma"], v0=w["v0"], k=w["k"], mu=w["mu"], tau=w["tau"], dt=w["dt"])
+    L = T.loss(0.8, 1.0, 0.1, T.target_field(4))
+    noise = T.gen_noise(2, 0.1, steps, n)
+    sim = T.Sim(mesh, p, n)
+    sim.stage_noise(noise)
+    sim.init_random(2)
+    sim.adjoint(steps, L, 0, ckpt)  # warm: graphs captured, buffers allocated
+    sim.init_random(2)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    code, g, dth, _, _ = sim.adjoint(steps, L, 0, ckpt)
+    elapsed = time.perf_counter() - t0
+    sim.close()
+    gpu_ms = g["forward_ms"] + g["reverse_ms"]
+    r = roofline(n, gpu_ms / 1e3 / steps, "checkpointed forward + recompute + reverse (one step)")
+    r["achieved"] *= 2  # forward + adjoint touch the state twice: 128 B per particle-step
+    r["frac"] *= 2
+    r["bytes_per_launch"] *= 2
+    return dict(workload=f"{n} particles, {steps} steps, checkpoints every {ckpt}", status=code,
+                value=n * steps / elapsed, unit="particle-steps/s (forward + adjoint)", elapsed_s=elapsed,
+                forward_ms=g["forward_ms"], reverse_ms=g["reverse_ms"], loss=g["loss"],
+                d_tau=g["d_tau"], d_mu=g["d_mu"], d_v0=g["d_v0"], d_k=g["d_k"],
+                max_abs_d_theta0=float(np.abs(dth).max()), roofline=r)
+
+
+def large(T, steps=500, adj_steps=200):
+    """The large surface: perturbed sphere r = 1 + 0.2 Y_3^2 (icosphere frequency 100, F = 200,000),
+    sigma = 0.01. 100,000 particles (seed 5, noise seed 5) for `steps` forward steps; then 64
+    replicas of 2,000 particles (tau log-uniform in [0.1, 10], seed 6) forward + adjoint together
+    over `adj_steps` steps with per-replica gradients."""
+    w = WORKLOAD
+    log("large surface: building the frequency-100 perturbed sphere")
+    t0 = time.perf_counter()
+    mesh = T.Mesh.perturbed_sphere(0.2, 100).build_tables(0.01)
+    build_s = time.perf_counter() - t0
+    info = mesh.info()
+    n = 100000
+    p = T.params(sigma=0.01, v0=w["v0"], k=w["k"], mu=w["mu"], tau=w["tau"], dt=w["dt"])
+    sim = T.Sim(mesh, p, n)
+    sim.init_random(5)
+    sim.stage_noise(T.gen_noise(5, 0.1, steps + 20, n))
+    sim.step_staged(20, 0)
+    sim.synchronize()
+    log(f"large surface: {n} particles, {steps} steps")
+    t0 = time.perf_counter()
+    code = sim.step_staged(steps, 20)
+    sim.synchronize()
+    el = time.perf_counter() - t0
+    fwd = dict(value=n * steps / el, unit="particle-steps/s", status=code, elapsed_s=el, launch=sim.launch(),
+               roofline=roofline(n, sim.last_step_ms() / 1e3 / steps, "k_step_tiled"))
+    sim.close()
+    R, nr = 64, 2000
+    rng = np.random.default_rng(6)
+    taus = np.exp(rng.uniform(np.log(0.1), np.log(10.0), R))
+    ps = [T.params(sigma=0.01, v0=w["v0"], k=w["k"], mu=w["mu"], tau=float(t), dt=w["dt"]) for t in taus]
+    seeds = np.arange(R, dtype=np.uint64) + 300
+    b = T.Batch(mesh, [nr] * R, ps)
+    b.gen_noise(seeds, np.full(R, 0.1), adj_steps)
+    L = T.loss(0.8, 1.0, 0.1, T.target_field(4))
+    b.init_random(seeds)
+    b.adjoint(adj_steps, L, 0, 20)  # warm
+    b.init_random(seeds)
+    log(f"large surface: {R} x {nr} forward + adjoint, {adj_steps} steps")
+    t0 = time.perf_counter()
+    code, gs, _, _, _ = b.adjoint(adj_steps, L, 0, 20)
+    el = time.perf_counter() - t0
+    b.close()
+    adj = dict(value=R * nr * adj_steps / el, unit="particle-steps/s (forward + adjoint)", status=code, elapsed_s=el,
+               forward_ms=gs[0]["forward_ms"], reverse_ms=gs[0]["reverse_ms"],
+               d_tau_first=[g["d_tau"] for g in gs[:4]])
+    return dict(mesh=dict(faces=info["nf"], uv_vertices=info["nv"], table_entries=info["nnz"], build_s=build_s),
+                forward=dict(workload=f"{n} particles, sigma 0.01, {steps} steps", **fwd),
+                batched_adjoint=dict(workload=f"{R} x {nr} particles, {adj_steps} steps, checkpoints every 20", **adj))
+
+
+def copy_calibration(mib: int, reps: int = 5) -> dict:
+    """A plain device-to-device copy of known size (torch's copy kernel), for scaling the
+    FETCH_SIZE / WRITE_SIZE counters of a profile to bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    return dict(bytes_per_copy=4 * n, copies=reps)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,11 +261,16 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=500)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run a device copy of MIB MiB (counter calibration for profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (replicas would shard "
               "across GPUs without per-step communication); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    calib = copy_calibration(args.copy_calibration) if args.copy_calibration > 0 else None
 
     T = importlib.import_module("2dtissue_amd")
     w = WORKLOAD
@@ -202,7 +304,7 @@ def main():
             "particles": n, "faces": tables["nf"], "uv_vertices": tables["nv"], "status": main_res["code"],
             "flags": main_res["flags"],
         },
-        "roofline": roofline(n, main_res["kernel_ms"] / 1e3 / args.steps, "k_step"),
+        "roofline": roofline(n, main_res["kernel_ms"] / 1e3 / args.steps, "k_step_tiled"),
         "plain_resident_kernel": {
             "value": n * args.steps / res_res["elapsed"], "unit": "particle-steps/s",
             "ms_per_step": res_res["elapsed"] * 1e3 / args.steps, "status": res_res["code"],
@@ -211,10 +313,17 @@ def main():
     }
     if not args.no_batched:
         out["batched"] = batched(T, mesh, args.batched_steps)
+    if not args.no_adjoint:
+        log("forward + adjoint: 1000 particles, 200 steps")
+        out["adjoint"] = adjoint(T, mesh)
+    if not args.no_large:
+        out["large"] = large(T)
     if not args.no_cpu:
         log(f"CPU baseline: {args.cpu_steps} oracle steps")
         out["cpu_baseline"] = cpu_baseline(tables, s0.x.astype(np.float64), s0.y.astype(np.float64),
                                            s0.theta.astype(np.float64), s0.vid, noise_h, args.cpu_steps)
+    if calib:
+        out["copy_calibration"] = calib
     print(json.dumps(out), flush=True)
 
 

@@ -171,6 +171,7 @@ struct StepScratch {
 extern "C" {
 
 void or_set_threads(int32_t n) { g_threads = n < 1 ? 1 : n; }
+int32_t or_get_threads(void) { return g_threads; }
 
 // LinearAlgebra::angles_to_unit_vectors, LinearAlgebra.cpp:25-46
 int32_t or_angles_to_unit_vectors(const int32_t* n_deg, int32_t n, double* out) {

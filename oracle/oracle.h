@@ -93,6 +93,17 @@ int32_t or_step_int_degrees(const or_mesh* m, const or_params* p, int32_t n, dou
 int32_t or_neighbor_sets(const or_mesh* m, const int32_t* vid, int32_t n, double sigma, int32_t* row_ptr /* n+1 */,
                          int32_t* cols, int32_t cap);
 
+int32_t or_get_threads(void);
+
+/* ---- the same step with a sparse neighbour search (oracle/sparse.cpp): identical operations in
+ * identical order (neighbours in ascending index, per-particle seam wrap), so bitwise equal to
+ * or_step with remap_mode 1, and O(N) per step instead of O(N^2) ---- */
+int32_t or_step_sparse(const or_mesh* m, const or_params* p, int32_t n, double* x, double* y, double* theta,
+                       int32_t* vid, const double* noise, double* vx, double* vy, double* r3d, int32_t* face,
+                       int32_t* nbr, double* phi);
+int32_t or_neighbor_sets_sparse(const or_mesh* m, const int32_t* vid, int32_t n, double sigma, int32_t* row_ptr,
+                                int32_t* cols, int32_t cap);
+
 /* ---- reverse pass (oracle/adjoint.cpp) ----
  * loss = w_order (phi_T - target_order)^2 + w_velocity mean_{t<T, i} |rdot_i^t - u(r_i^t)|^2,
  * u_c(x, y) = sum over modes with component c of amp sin(kx x + ky y + phase) */

@@ -100,6 +100,33 @@ def step(mesh: Mesh, p: dict, x, y, theta, vid, noise=None, remap_mode=1):
                                                     phi_polar=phi[1])
 
 
+def step_sparse(mesh: Mesh, p: dict, x, y, theta, vid, noise=None):
+    """or_step_sparse: the same step as step(..., remap_mode=1), O(N) per step."""
+    n = len(x)
+    x = np.array(x, np.float64); y = np.array(y, np.float64); th = np.array(theta, np.float64)
+    v = np.array(vid, np.int32)
+    nz = None if noise is None else np.ascontiguousarray(noise, np.float64)
+    vx = np.zeros(n); vy = np.zeros(n); r3d = np.zeros((n, 3)); face = np.zeros(n, np.int32)
+    nbr = np.zeros(n, np.int32); phi = np.zeros(2)
+    prm = OrParams(p["sigma"], p["v0"], p["k"], p["mu"], p["tau"], p["dt"], p.get("max_crossings", 0) or 16384)
+    st = lib().or_step_sparse(mesh.ref, C.byref(prm), n, dp(x), dp(y), dp(th), ip(v), None if nz is None else dp(nz),
+                              dp(vx), dp(vy), dp(r3d), ip(face), ip(nbr), dp(phi))
+    return st, dict(x=x, y=y, theta=th, vid=v), dict(vx=vx, vy=vy, r3d=r3d, face=face, nbr=nbr, phi_ref=phi[0],
+                                                    phi_polar=phi[1])
+
+
+def neighbor_sets_sparse(mesh: Mesh, vid, sigma):
+    n = len(vid)
+    v = np.ascontiguousarray(vid, np.int32)
+    cap = max(64 * n, 1024)
+    while True:
+        rp = np.zeros(n + 1, np.int32); cols = np.zeros(cap, np.int32)
+        st = lib().or_neighbor_sets_sparse(mesh.ref, ip(v), n, C.c_double(sigma), ip(rp), ip(cols), cap)
+        if st == OK:
+            return [cols[rp[i]:rp[i + 1]].copy() for i in range(n)]
+        cap *= 4
+
+
 def step_int_degrees(mesh: Mesh, p: dict, x, y, n_deg, vid, remap_mode=1):
     n = len(x)
     x = np.array(x, np.float64); y = np.array(y, np.float64)

@@ -1,0 +1,75 @@
+"""The large surface: perturbed sphere r = 1 + 0.2 Y_3^2 at icosphere frequency 100 (F = 200,000),
+sigma = 0.01. 100,000 particles teacher-forced against the oracle's O(N) step (bitwise equal to
+the reference-literal dense step, see test_oracle.py), and 64 replicas of 2,000 particles run
+forward and reverse together, each replica's gradients against the FP64 reverse pass along its
+own trajectory."""
+import numpy as np
+import pytest
+
+import parity as P
+
+pytestmark = pytest.mark.gpu
+
+
+def test_large_surface_teacher_forced(T, O):
+    cfg = P.C5
+    m, tb = P.sphere_and_tables(cfg["freq"], cfg["amp"], cfg["sigma"])
+    om = O.Mesh(tb)
+    O.set_threads(16)
+    p = dict(P.PHYS, sigma=cfg["sigma"])
+    sim = T.Sim(m, T.params(**p), cfg["n"])
+    sim.init_random(cfg["seed"])
+    noise = T.gen_noise(cfg["noise_seed"], 0.1, 4, cfg["n"])
+    assert sim.step(2, noise[:2]) == T.OK  # leave the initial placement first
+    total = {}
+    for t in range(2, 4):
+        s_in = sim.get_state()
+        nb = sim.neighbor_lists()
+        assert sim.step(1, noise[t]) == T.OK
+        fails, _ = P.compare_step(O, om, p, s_in, sim.get_state(), sim.get_outputs(), nb,
+                                  noise[t].astype(np.float64), sparse=True)
+        print(t, fails, flush=True)
+        for k, v in fails.items():
+            total[k] = total.get(k, 0) + v
+    assert all(v == 0 for v in total.values()), total
+
+
+def test_large_surface_batched_adjoint(T, O):
+    cfg = P.C5
+    m, tb = P.sphere_and_tables(cfg["freq"], cfg["amp"], cfg["sigma"])
+    om = O.Mesh(tb)
+    R, n, steps = 64, 2000, 10
+    rng = np.random.default_rng(6)
+    taus = np.exp(rng.uniform(np.log(0.1), np.log(10.0), R))
+    ps = [T.params(sigma=cfg["sigma"], v0=1.0, k=10.0, mu=1.0, tau=float(t), dt=0.02) for t in taus]
+    seeds = np.arange(R, dtype=np.uint64) + 300
+    b = T.Batch(m, [n] * R, ps)
+    b.init_random(seeds)
+    b.gen_noise(seeds, np.full(R, 0.1), steps)
+    L = T.loss(0.8, 1.0, 0.1, T.target_field(4))
+    code, gs, dth, _, _ = b.adjoint(steps, L, 0, 5)
+    assert code == T.OK
+    modes = T.target_field(4)
+    comp, amp, kx, ky, ph = (np.array(v) for v in zip(*modes))
+    Lo = O.Loss(0.8, 1.0, 0.1, component=comp, amp=amp, kx=kx, ky=ky, phase=ph)
+    for r in (0, 17, 63):
+        s = T.Sim(m, ps[r], n)
+        s.set_mode(T.MODE_STEP_KERNEL)
+        s.init_random(int(seeds[r]))
+        eta = T.gen_noise(int(seeds[r]), 0.1, steps, n)
+        xs, ys, ths, vids = [], [], [], []
+        for t in range(steps + 1):
+            st = s.get_state()
+            xs.append(st.x); ys.append(st.y); ths.append(st.theta); vids.append(st.vid)
+            if t < steps:
+                assert s.step(1, eta[t]) == T.OK
+        states = dict(x=np.array(xs, np.float64), y=np.array(ys, np.float64), theta=np.array(ths, np.float64),
+                      vid=np.array(vids, np.int32))
+        pr = dict(sigma=cfg["sigma"], v0=1.0, k=10.0, mu=1.0, tau=float(taus[r]), dt=0.02)
+        ref = O.adjoint_reverse(om, pr, states, Lo)
+        for key in ("d_tau", "d_mu", "d_v0"):
+            assert abs(gs[r][key] - ref[key]) <= 1e-4 * abs(ref[key]), (r, key, gs[r][key], ref[key])
+        o = r * n
+        d = dth[o:o + n]
+        assert np.linalg.norm(d - ref["d_theta0"]) <= 1e-4 * np.linalg.norm(ref["d_theta0"]), r
+        s.close()

@@ -297,3 +297,30 @@ def test_step_nonfinite_reported(O, mesh1):
     st, *_ = O.step(om, p, [0.5, 0.52], [0.5, 0.5], [0.0, 1.0], O.remap(om, [0.5, 0.52], [0.5, 0.5], 0)[1],
                     [float("nan"), 0.0])
     assert st == O.E_NONFINITE
+
+
+@pytest.mark.parametrize("freq,sigma,n,steps", [(22, 0.1, 100, 20), (32, 0.05, 2000, 3)])
+def test_sparse_step_equals_dense_step(O, freq, sigma, n, steps):
+    """or_step_sparse (neighbours from the geodesic rows, per-particle seam wrap) reproduces the
+    reference-literal dense or_step bit for bit, so the large-surface parity checks that use it
+    inherit the dense step's pinning."""
+    import parity as P
+    _, tb = P.mesh_and_tables(freq, sigma)
+    om = O.Mesh(tb)
+    rng = np.random.default_rng(3)
+    x = rng.uniform(0.02, 0.98, n); y = rng.uniform(0.02, 0.98, n); th = rng.uniform(-np.pi, np.pi, n)
+    _, vid, _ = O.remap(om, x, y)
+    p = dict(P.PHYS, sigma=sigma)
+    a = dict(x=x, y=y, theta=th, vid=vid)
+    for t in range(steps):
+        eta = rng.normal(0, 0.1, n)
+        assert [list(s) for s in O.neighbor_sets_sparse(om, a["vid"], sigma)] == \
+            [list(s) for s in O.neighbor_sets(om, a["vid"], sigma)]
+        st1, s1, o1 = O.step(om, p, a["x"], a["y"], a["theta"], a["vid"], eta, remap_mode=1)
+        st2, s2, o2 = O.step_sparse(om, p, a["x"], a["y"], a["theta"], a["vid"], eta)
+        assert st1 == st2
+        for k in ("x", "y", "theta", "vid"):
+            assert np.array_equal(s1[k], s2[k]), (t, k)
+        for k in ("vx", "vy", "r3d", "face", "nbr", "phi_ref", "phi_polar"):
+            assert np.array_equal(o1[k], o2[k]), (t, k)
+        a = s1
