@@ -33,10 +33,14 @@ struct StepParams {
 // an FP32 speed carries ~1e-7 relative error, which a displacement D turns into ~1e-7 D absolute
 constexpr float kBigStep = 2.0f;
 
-// vertex buckets of one step: count[v], the first kSlots ids in slot[v*kSlots..], the rest as
-// (vertex, id) pairs in the overflow list
-struct BucketsIn { const int* count; const int* slot; const int* ovf_n; const int2* ovf; };
-struct BucketsOut { int* count; int* slot; int* ovf_n; int2* ovf; };
+// Vertex buckets of one step, read after k_bucket_finalize has run on them: count[v] particles;
+// up to kSlots of them in slot[v*kSlots ..] in ascending id order; a bucket with more holds its
+// ids, ascending, in big[slot[v*kSlots] ..] instead.
+struct BucketsIn { const int* count; const int* slot; const int* big; };
+// The same buckets while the step fills them: the first kSlots arrivals of a vertex take its
+// slots, later ones go to the (vertex, id) overflow list; the second arrival at a vertex also
+// lists the vertex in multi (the buckets the finalize pass must sort).
+struct BucketsOut { int* count; int* slot; int* ovf_n; int2* ovf; int* multi_n; int* multi; };
 
 // ------------------------------------------------------------------ neighbour traversal
 
@@ -46,28 +50,12 @@ __device__ __forceinline__ void for_bucket(const B& b, int w, F&& f) {
   const int c = b.count[w];
   if (c == 0) return;
   const int* s = b.slot + (size_t)w * kSlots;
-  if (c == 1) { f(s[0]); return; }
-  if (c == 2) {
-    const int a = s[0], d = s[1];
-    if (a < d) { f(a); f(d); } else { f(d); f(a); }
+  if (c <= kSlots) {
+    for (int q = 0; q < c; ++q) f(s[q]);
     return;
   }
-  const int ns = c < kSlots ? c : kSlots;
-  const int novf = c > kSlots ? *b.ovf_n : 0;
-  int prev = -1;
-  for (int r = 0; r < c; ++r) {
-    int best = 0x7fffffff;
-    for (int q = 0; q < ns; ++q) {
-      const int id = s[q];
-      if (id > prev && id < best) best = id;
-    }
-    for (int q = 0; q < novf; ++q) {
-      const int2 e = b.ovf[q];
-      if (e.x == w && e.y > prev && e.y < best) best = e.y;
-    }
-    f(best);
-    prev = best;
-  }
+  const int* g = b.big + s[0];
+  for (int q = 0; q < c; ++q) f(g[q]);
 }
 
 // force coefficient Fij / d for a pair at table distance d < 2 sigma
@@ -123,10 +111,7 @@ struct GlobalSource {
   __device__ __forceinline__ void each(int w, const Head& h, F&& f) const {
     if (h.c == 0) return;
     if (h.c == 1) { f(h.s0); return; }
-    if (h.c == 2) {
-      if (h.s0 < h.s1) { f(h.s0); f(h.s1); } else { f(h.s1); f(h.s0); }
-      return;
-    }
+    if (h.c == 2) { f(h.s0); f(h.s1); return; }  // sorted by the finalize pass
     for_bucket(b, w, f);
   }
   __device__ __forceinline__ float4 pos(int j) const { return xycs[j]; }

@@ -110,7 +110,10 @@ void Engine::release() {
   for (int k = 0; k < 2; ++k) { dfree(xycs[k]); dfree(th[k]); dfree(vid[k]); }
   dfree(vx); dfree(vy); dfree(r3x); dfree(r3y); dfree(r3z); dfree(face); dfree(nbr);
   dfree(d_halt); dfree(d_rows); dfree(d_first_bad);
-  for (int k = 0; k < 3; ++k) { dfree(bs[k].count); dfree(bs[k].slot); dfree(bs[k].ovf_n); dfree(bs[k].ovf); }
+  for (int k = 0; k < 3; ++k) {
+    dfree(bs[k].count); dfree(bs[k].slot); dfree(bs[k].ovf_n); dfree(bs[k].ovf);
+    dfree(bs[k].multi_n); dfree(bs[k].multi); dfree(bs[k].big_n); dfree(bs[k].big);
+  }
   dfree(d_tiles); dfree(d_tile_begin); dfree(partial); dfree(phi); dfree(noise); dfree(call_noise);
   adjoint_release();
   if (ev0) (void)hipEventDestroy(ev0);
@@ -203,9 +206,25 @@ void Engine::ensure_buckets() {
     bs[k].slot = dalloc<int>(cells * tdev::kSlots);
     bs[k].ovf_n = dalloc<int>(R);
     bs[k].ovf = dalloc<int2>(ntot);
+    bs[k].multi_n = dalloc<int>(R);
+    bs[k].multi = dalloc<int>(ntot);
+    bs[k].big_n = dalloc<int>(R);
+    bs[k].big = dalloc<int>(ntot);
   }
   buckets_ready = true;
   buckets_stale = true;
+}
+
+// outside any capture: clear set k, bucket the vertex ids v into it and sort its buckets
+void Engine::bucket_state(const int* v, int k) {
+  if (!d_tiles) ensure_tiles(resolve_tiled());
+  const size_t cells = (size_t)R * md.nv;
+  HIP_OK(hipMemsetAsync(bs[k].count, 0, cells * sizeof(int), stream));
+  HIP_OK(hipMemsetAsync(bs[k].ovf_n, 0, R * sizeof(int), stream));
+  HIP_OK(hipMemsetAsync(bs[k].multi_n, 0, R * sizeof(int), stream));
+  HIP_OK(hipMemsetAsync(bs[k].big_n, 0, R * sizeof(int), stream));
+  HIP_OK(launch_insert(ntot, v, d_rid, d_reps, md.nv, bs[k], stream));
+  HIP_OK(launch_bucket_finalize(std::get<1>(tiles_key), d_reps, d_tiles, n_phys, md.nv, bs[k], stream));
 }
 
 // outside any capture: clear every set, bucket the current state into set bphase
@@ -215,8 +234,10 @@ void Engine::rebuild_buckets() {
   for (int k = 0; k < 3; ++k) {
     HIP_OK(hipMemsetAsync(bs[k].count, 0, cells * sizeof(int), stream));
     HIP_OK(hipMemsetAsync(bs[k].ovf_n, 0, R * sizeof(int), stream));
+    HIP_OK(hipMemsetAsync(bs[k].multi_n, 0, R * sizeof(int), stream));
+    HIP_OK(hipMemsetAsync(bs[k].big_n, 0, R * sizeof(int), stream));
   }
-  HIP_OK(launch_insert(ntot, vid[sphase], d_rid, d_reps, md.nv, bs[bphase], stream));
+  bucket_state(vid[sphase], bphase);
   buckets_stale = false;
 }
 
@@ -406,6 +427,7 @@ void Engine::enqueue(const Shape& s, int n_steps, const float* nz, size_t stride
         io.step = t;
         HIP_OK(launch_step_tiled(s.lanes, s.block, false, md, d_reps, d_tiles, n_phys, bs[bp], bs[(bp + 1) % 3],
                                  bs[(bp + 2) % 3], io, stream));
+        HIP_OK(launch_bucket_finalize(s.block, d_reps, d_tiles, n_phys, md.nv, bs[(bp + 1) % 3], stream));
         sp = 1 - sp;
         bp = (bp + 1) % 3;
       }

@@ -132,6 +132,7 @@ struct Engine {
   void release();
   void ensure_buckets();
   void rebuild_buckets();
+  void bucket_state(const int* vid, int set);
   void ensure_tiles(const Shape& s);
   void enqueue(const Shape& s, int n_steps, const float* noise, size_t stride);
   Shape resolve_for(Shape s) const;

@@ -128,22 +128,19 @@ int32_t Engine::adjoint(int T, const float* nz, size_t stride, int K, const tiss
   HIP_OK(hipMemsetAsync(a.sexp, 0, R * sizeof(int), stream));
   int cur = 0;
   const size_t cells = (size_t)R * md.nv;
-  auto bucket_state = [&](const int* v, int set) {
-    HIP_OK(hipMemsetAsync(bs[set].count, 0, cells * sizeof(int), stream));
-    HIP_OK(hipMemsetAsync(bs[set].ovf_n, 0, R * sizeof(int), stream));
-    HIP_OK(launch_insert(ntot, v, d_rid, d_reps, md.nv, bs[set], stream));
-  };
   for (int c = C - 1; c >= 0; --c) {
     const int t0 = c * K, kc = std::min(K, T - t0);
     // recompute the segment from its checkpoint, recording the tape
     HIP_OK(hipMemcpyAsync(a.tp_xycs, a.ck_xycs + c * N, N * sizeof(float4), hipMemcpyDeviceToDevice, stream));
     HIP_OK(hipMemcpyAsync(a.tp_th, a.ck_th + c * N, N * sizeof(float), hipMemcpyDeviceToDevice, stream));
     HIP_OK(hipMemcpyAsync(a.tp_vid, a.ck_vid + c * N, N * sizeof(int), hipMemcpyDeviceToDevice, stream));
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 1; k < 3; ++k) {
       HIP_OK(hipMemsetAsync(bs[k].count, 0, cells * sizeof(int), stream));
       HIP_OK(hipMemsetAsync(bs[k].ovf_n, 0, R * sizeof(int), stream));
+      HIP_OK(hipMemsetAsync(bs[k].multi_n, 0, R * sizeof(int), stream));
+      HIP_OK(hipMemsetAsync(bs[k].big_n, 0, R * sizeof(int), stream));
     }
-    HIP_OK(launch_insert(ntot, a.tp_vid, d_rid, d_reps, md.nv, bs[0], stream));
+    bucket_state(a.tp_vid, 0);
     HIP_OK(launch_flags_begin(R, d_halt, a.rows, a.first_bad, stream));
     for (int q = 0; q < kc; ++q) {
       TiledIO io{};
@@ -161,6 +158,7 @@ int32_t Engine::adjoint(int T, const float* nz, size_t stride, int K, const tiss
       io.tape_q = a.tp_q + q * N;
       HIP_OK(launch_step_tiled(s.lanes, s.block, true, md, d_reps, d_tiles, n_phys, bs[q % 3], bs[(q + 1) % 3],
                                bs[(q + 2) % 3], io, stream));
+      HIP_OK(launch_bucket_finalize(s.block, d_reps, d_tiles, n_phys, md.nv, bs[(q + 1) % 3], stream));
     }
     // backwards through the segment
     for (int q = kc - 1; q >= 0; --q) {

@@ -39,6 +39,20 @@ struct MeshDev {
 __host__ __device__ inline bool inside_unit(double x, double y) { return x >= 0.0 && x <= 1.0 && y >= 0.0 && y <= 1.0; }
 
 // fl(n / d) in [0, 1], decided without dividing (d != 0)
+// ascending sort of 8 values in registers: Batcher's odd-even merge network (19 comparators,
+// fixed indices, so the values stay in registers)
+__host__ __device__ inline void sort8(int (&a)[8]) {
+  constexpr int P[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6}, {5, 7}, {1, 2}, {5, 6},
+                            {0, 4}, {1, 5}, {2, 6}, {3, 7}, {2, 4}, {3, 5}, {1, 2}, {3, 4}, {5, 6}};
+#pragma unroll
+  for (int k = 0; k < 19; ++k) {
+    const int lo = a[P[k][0]] < a[P[k][1]] ? a[P[k][0]] : a[P[k][1]];
+    const int hi = a[P[k][0]] < a[P[k][1]] ? a[P[k][1]] : a[P[k][0]];
+    a[P[k][0]] = lo;
+    a[P[k][1]] = hi;
+  }
+}
+
 __host__ __device__ inline bool quotient_in_unit(double n, double d) {
   return (n == 0.0 || ((n > 0.0) == (d > 0.0))) && fabs(n) <= fabs(d);
 }

@@ -28,14 +28,22 @@ __device__ __forceinline__ void bucket_insert(const BucketsOut& b, int v, int i)
     const int o = atomicAdd(b.ovf_n, 1);
     b.ovf[o] = make_int2(v, i);
   }
+  if (old == 1) b.multi[atomicAdd(b.multi_n, 1)] = v;  // the bucket needs sorting
 }
 
 // buckets of replica r within a set
 __device__ __forceinline__ BucketsIn replica_in(const BucketSet& s, int r, int nv, int off) {
-  return {s.count + (size_t)r * nv, s.slot + (size_t)r * nv * kSlots, s.ovf_n + r, s.ovf + off};
+  return {s.count + (size_t)r * nv, s.slot + (size_t)r * nv * kSlots, s.big + off};
 }
 __device__ __forceinline__ BucketsOut replica_out(const BucketSet& s, int r, int nv, int off) {
-  return {s.count + (size_t)r * nv, s.slot + (size_t)r * nv * kSlots, s.ovf_n + r, s.ovf + off};
+  return {s.count + (size_t)r * nv, s.slot + (size_t)r * nv * kSlots, s.ovf_n + r, s.ovf + off, s.multi_n + r,
+          s.multi + off};
+}
+
+// orders the LDS / global accesses of one wave (its lanes exchange data through memory)
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
 }
 
 // block-wide deterministic sum of (sgn, cs, sn): butterfly within each wave64, then wave
@@ -76,6 +84,8 @@ __global__ void __launch_bounds__(kStepMaxBlock) k_step_tiled(MeshDev mesh, cons
   const unsigned halt = io.halt_in[r];
   if (tile.q == 0 && tid == 0) {
     bclr.ovf_n[r] = 0;
+    bclr.multi_n[r] = 0;
+    bclr.big_n[r] = 0;
     io.halt_clr[r] = 0u;
     if (halt) atomicOr(&io.halt_out[r], halt);
   }
@@ -128,6 +138,88 @@ __global__ void __launch_bounds__(kStepMaxBlock) k_step_tiled(MeshDev mesh, cons
     }
   }
   block_order_partial(sgn, pcs, psn, io.partial + tile.logical);
+}
+
+// crowded buckets up to this many particles are sorted in LDS (larger ones in global memory)
+constexpr int kSortCap = 512;
+
+// Sort every bucket of set bs that holds two or more particles (listed in multi by the insertion).
+// One thread per listed bucket: up to kSlots ids are sorted in place with the register network.
+// A crowded bucket (more than kSlots) is then taken by its whole wave: its slots and its entries
+// of the overflow list are gathered (ballot compaction), ranked, and written in ascending order to
+// a fresh range of the replica's big area, whose offset replaces the bucket's first slot.
+__global__ void __launch_bounds__(kStepMaxBlock) k_bucket_finalize(const ReplicaDesc* reps, const TileDesc* tiles,
+                                                                   int nv, BucketSet bs) {
+  __shared__ int lds[kStepMaxBlock / 64][kSortCap];
+  const TileDesc tile = tiles[blockIdx.x];
+  if (tile.r < 0) return;
+  const int r = tile.r;
+  const int off = reps[r].off;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int* slot = bs.slot + (size_t)r * nv * kSlots;
+  const int k = tile.q * (int)blockDim.x + tid;
+  int v = -1, c = 0;
+  if (k < bs.multi_n[r]) {
+    v = bs.multi[off + k];
+    c = bs.count[(size_t)r * nv + v];
+    if (c <= kSlots) {
+      int* sl = slot + (size_t)v * kSlots;
+      int a[kSlots];
+#pragma unroll
+      for (int q = 0; q < kSlots; ++q) a[q] = q < c ? sl[q] : 0x7fffffff;
+      sort8(a);
+#pragma unroll
+      for (int q = 0; q < kSlots; ++q)
+        if (q < c) sl[q] = a[q];
+    }
+  }
+  unsigned long long crowded = __ballot(c > kSlots);
+  if (crowded == 0ull) return;  // uniform over the wave
+  const int novf = bs.ovf_n[r];
+  const int2* ovf = bs.ovf + off;
+  int* big = bs.big + off;
+  while (crowded) {
+    const int src = __ffsll((long long)crowded) - 1;
+    crowded &= crowded - 1ull;
+    const int w = __shfl(v, src, 64);
+    const int cw = __shfl(c, src, 64);
+    int base = 0;
+    if (lane == 0) base = atomicAdd(&bs.big_n[r], cw);
+    base = __shfl(base, 0, 64);
+    int* sl = slot + (size_t)w * kSlots;
+    int* dst = big + base;
+    const bool in_lds = cw <= kSortCap;
+    int* buf = in_lds ? lds[wave] : dst;
+    if (lane < kSlots) buf[lane] = sl[lane];
+    int fill = kSlots;
+    for (int q0 = 0; q0 < novf; q0 += 64) {
+      const int q = q0 + lane;
+      const int2 e = q < novf ? ovf[q] : make_int2(-1, 0);
+      const bool m = e.x == w;
+      const unsigned long long bm = __ballot(m);
+      if (m) buf[fill + __popcll(bm & ((1ull << lane) - 1ull))] = e.y;
+      fill += __popcll(bm);
+    }
+    wave_sync();
+    if (in_lds) {
+      // rank of each id among the bucket's (distinct) ids = its position in ascending order
+      for (int q = lane; q < cw; q += 64) {
+        const int val = buf[q];
+        int rank = 0;
+        for (int j = 0; j < cw; ++j) rank += buf[j] < val ? 1 : 0;
+        dst[rank] = val;
+      }
+    } else if (lane == 0) {
+      for (int a = 1; a < cw; ++a) {  // rare: insertion sort in place in global memory
+        const int val = dst[a];
+        int b = a - 1;
+        while (b >= 0 && dst[b] > val) { dst[b + 1] = dst[b]; --b; }
+        dst[b + 1] = val;
+      }
+    }
+    wave_sync();
+    if (lane == 0) sl[0] = base;
+  }
 }
 
 __global__ void k_phi_tiles(int R, int steps, const int* tile_begin, int n_logical, const OrderPartial* partial,
@@ -238,6 +330,12 @@ hipError_t launch_step_tiled(int lanes, int block, bool tape, const MeshDev& mes
     case 16: launch_tiled_lanes<16>(block, tape, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_bucket_finalize(int block, const ReplicaDesc* reps, const TileDesc* tiles, int n_tiles, int nv,
+                                  const BucketSet& bs, hipStream_t st) {
+  hipLaunchKernelGGL(k_bucket_finalize, dim3(n_tiles), dim3(block), 0, st, reps, tiles, nv, bs);
   return hipGetLastError();
 }
 

@@ -28,12 +28,17 @@ struct TileDesc {
 };
 
 // one vertex-bucket set of all replicas: replica r owns count[r*nv ..], slot[r*nv*kSlots ..],
-// ovf_n[r] and the overflow entries ovf[off_r .. off_r + n_r)
+// the counters ovf_n[r], multi_n[r], big_n[r] and the entries ovf, multi and big at
+// [off_r .. off_r + n_r)
 struct BucketSet {
   int* count;
   int* slot;
   int* ovf_n;
   int2* ovf;
+  int* multi_n;
+  int* multi;
+  int* big_n;
+  int* big;
 };
 
 // everything one step of the tiled kernel reads and writes besides the mesh and the buckets
@@ -109,6 +114,10 @@ hipError_t launch_resident(int lanes, int block, const tdev::MeshDev& mesh, cons
 hipError_t launch_step_tiled(int lanes, int block, bool tape, const tdev::MeshDev& mesh, const ReplicaDesc* reps,
                              const TileDesc* tiles, int n_tiles, const BucketSet& bin, const BucketSet& bout,
                              const BucketSet& bclr, const TiledIO& io, hipStream_t st);
+// sort the buckets of set bs that hold two or more particles (over the same tiles as the step);
+// crowded buckets (more than kSlots) move, sorted, into the big area
+hipError_t launch_bucket_finalize(int block, const ReplicaDesc* reps, const TileDesc* tiles, int n_tiles, int nv,
+                                  const BucketSet& bs, hipStream_t st);
 // order parameters of `steps` steps of every replica from the tiles' partials
 // (partial[s * n_logical + tile]); replica r's tiles are tile_begin[r] .. tile_begin[r+1]
 hipError_t launch_phi_tiles(int R, int steps, const int* tile_begin, int n_logical, const OrderPartial* partial,

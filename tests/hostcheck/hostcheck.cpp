@@ -7,6 +7,15 @@ extern "C" {
 
 int hc_quotient_in_unit(double n, double d) { return tdev::quotient_in_unit(n, d) ? 1 : 0; }
 
+void hc_sort8_many(int n, int* v /* n*8, sorted in place */) {
+  for (int i = 0; i < n; ++i) {
+    int a[8];
+    for (int k = 0; k < 8; ++k) a[k] = v[8 * i + k];
+    tdev::sort8(a);
+    for (int k = 0; k < 8; ++k) v[8 * i + k] = a[k];
+  }
+}
+
 void hc_seam_wrap_many(int n, const double* a, const double* p, int max_crossings, double* out_p, double* dth,
                        int* ok, int* crossings) {
   for (int i = 0; i < n; ++i) {

@@ -146,3 +146,19 @@ def test_locate_matches_oracle_near_edges_and_vertices(H, O, mesh32):
     assert np.array_equal(face, of) and np.array_equal(vid, ov)
     assert np.abs(r3d - r3).max() < 1e-5
     assert (fp & 1).any() and (fp & 2).any()  # the FP64 fallbacks are exercised
+
+
+def test_sort8_network(H):
+    """The bucket sort network sorts every 0-1 input of length 8 (0-1 principle: then it sorts
+    every input), and random and padded (INT_MAX) inputs as the buckets use it."""
+    rows = np.array([[(m >> k) & 1 for k in range(8)] for m in range(256)], np.int32)
+    rng = np.random.default_rng(5)
+    rand = rng.integers(0, 1 << 30, (2000, 8)).astype(np.int32)
+    pad = rand.copy()
+    c = rng.integers(0, 9, 2000)
+    for i, k in enumerate(c):
+        pad[i, k:] = np.iinfo(np.int32).max
+    for v in (rows, rand, pad):
+        a = np.ascontiguousarray(v.copy())
+        H.hc_sort8_many(len(a), p(a, C.c_int32))
+        assert np.array_equal(a, np.sort(v, axis=1))
