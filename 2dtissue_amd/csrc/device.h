@@ -163,14 +163,15 @@ struct LdsSource {
 // Neighbours: particles bucketed on the vertices w of row v of the geodesic table; pair
 // distance d = D[v, w]. Force and alignment for d < 2 sigma (ForceHelper.cpp:34-73,
 // OrientationHelper.cpp:29-74, self included in the mean), count for 0 < d <= 2.4 sigma
-// (2DTissue.cpp:254-268). FP32 sums; FP64 re-evaluation when the displacement is large or the
-// alignment sum nearly cancels.
+// (2DTissue.cpp:254-268). FP32 sums, with the force also summed in FP64 (used when the
+// displacement is large); the alignment sum is redone in FP64 when it nearly cancels.
 // Always inlined: as an out-of-line call (which the inliner picks once two kernels share an
 // instantiation) its result and live state go through scratch memory.
 template <int LANES, class Src>
 __device__ __forceinline__ StepOut particle_step(const MeshDev& mesh, const StepParams& prm, const Src& src, int i, float4 me,
                                         float th, int v, float eta, int lane) {
   float Fx = 0.f, Fy = 0.f, Sx = 0.f, Sy = 0.f;
+  double Fx64 = 0.0, Fy64 = 0.0;  // the same force in FP64 (used when the displacement is large)
   int na = 0, nb = 0;
   const int rs = mesh.row_ptr[v], re = mesh.row_ptr[v + 1];
   // Each lane walks its entries rs + lane + m * LANES in order, kRowChunk at a time: the chunk's
@@ -197,12 +198,16 @@ __device__ __forceinline__ StepOut particle_step(const MeshDev& mesh, const Step
         continue;
       }
       const float g = pair_coef(d, prm);
+      const double dd = d == 0.0f ? 0.001 : (double)d;
+      const double g64 = ((-prm.k64 * (prm.two_sigma64 - dd)) / prm.two_sigma64) / dd;
       const bool counted = d != 0.0f;
       src.each(ent[k].x, head[k], [&](int j) {
         const float4 q = src.pos(j);
         if (j != i) {
           Fx += g * (me.x - q.x);
           Fy += g * (me.y - q.y);
+          Fx64 += g64 * ((double)me.x - (double)q.x);
+          Fy64 += g64 * ((double)me.y - (double)q.y);
         }
         Sx += q.z;
         Sy += q.w;
@@ -231,8 +236,23 @@ __device__ __forceinline__ StepOut particle_step(const MeshDev& mesh, const Step
   const bool weak = !(Sm >= 0.6f * prm.rate * (float)na);
   const bool big = !(spd * prm.dt <= kBigStep);
   double thn;
-  if (weak || big) {
-    double Fx64 = 0.0, Fy64 = 0.0, Sx64 = 0.0, Sy64 = 0.0;
+  if (big) {  // uniform over the particle's lanes: the FP32 sums were reduced over the group
+    Fx64 = group_sum<LANES>(Fx64);
+    Fy64 = group_sum<LANES>(Fy64);
+    double st, ct;
+    sincos((double)th, &st, &ct);
+    const double spd64 = prm.v064 + prm.mu64 * sqrt(Fx64 * Fx64 + Fy64 * Fy64);
+    const double vx64 = ct * spd64, vy64 = st * spd64;
+    xn = (double)me.x + vx64 * prm.dt64;
+    yn = (double)me.y + vy64 * prm.dt64;
+    vx = (float)vx64;
+    vy = (float)vy64;
+    Fx = (float)Fx64;
+    Fy = (float)Fy64;
+  }
+  if (weak) {
+    // the alignment sum again, over the neighbours' headings in FP64
+    double Sx64 = 0.0, Sy64 = 0.0;
     for (int base = rs + lane; base < re; base += kRowChunk * LANES) {
       int2 ent[kRowChunk];
       typename Src::Head head[kRowChunk];
@@ -249,39 +269,16 @@ __device__ __forceinline__ StepOut particle_step(const MeshDev& mesh, const Step
       for (int k = 0; k < kRowChunk; ++k) {
         const float d = __int_as_float(ent[k].y);
         if (d > prm.t_near) { stop = true; break; }
-        const double dd = d == 0.0f ? 0.001 : (double)d;
-        const double fij = (-prm.k64 * (prm.two_sigma64 - dd)) / prm.two_sigma64;
         src.each(ent[k].x, head[k], [&](int j) {
-          if (big && j != i) {
-            const float4 q = src.pos(j);
-            Fx64 += fij * (((double)me.x - (double)q.x) / dd);
-            Fy64 += fij * (((double)me.y - (double)q.y) / dd);
-          }
-          if (weak) {
-            double sj, cj;
-            sincos((double)src.theta(j), &sj, &cj);
-            Sx64 += cj;
-            Sy64 += sj;
-          }
+          double sj, cj;
+          sincos((double)src.theta(j), &sj, &cj);
+          Sx64 += cj;
+          Sy64 += sj;
         });
       }
       if (stop) break;
     }
-    if (big) {
-      Fx64 = group_sum<LANES>(Fx64);
-      Fy64 = group_sum<LANES>(Fy64);
-      double st, ct;
-      sincos((double)th, &st, &ct);
-      const double spd64 = prm.v064 + prm.mu64 * sqrt(Fx64 * Fx64 + Fy64 * Fy64);
-      const double vx64 = ct * spd64, vy64 = st * spd64;
-      xn = (double)me.x + vx64 * prm.dt64;
-      yn = (double)me.y + vy64 * prm.dt64;
-      vx = (float)vx64;
-      vy = (float)vy64;
-      Fx = (float)Fx64;
-      Fy = (float)Fy64;
-    }
-    if (weak) {
+    {
       Sx64 = group_sum<LANES>(Sx64);
       Sy64 = group_sum<LANES>(Sy64);
       const double mean = atan2(Sy64, Sx64);

@@ -38,7 +38,6 @@ struct MeshDev {
 
 __host__ __device__ inline bool inside_unit(double x, double y) { return x >= 0.0 && x <= 1.0 && y >= 0.0 && y <= 1.0; }
 
-// fl(n / d) in [0, 1], decided without dividing (d != 0)
 // ascending sort of 8 values in registers: Batcher's odd-even merge network (19 comparators,
 // fixed indices, so the values stay in registers)
 __host__ __device__ inline void sort8(int (&a)[8]) {
@@ -53,62 +52,22 @@ __host__ __device__ inline void sort8(int (&a)[8]) {
   }
 }
 
+// fl(n / d) in [0, 1], decided without dividing (d != 0)
 __host__ __device__ inline bool quotient_in_unit(double n, double d) {
   return (n == 0.0 || ((n > 0.0) == (d > 0.0))) && fabs(n) <= fabs(d);
 }
 
+// conjunction without short circuit: every operand is evaluated
+__host__ __device__ inline bool all_of4(bool a, bool b, bool c, bool d) { return ((int)a & (int)b & (int)c & (int)d) != 0; }
+
 // EuclideanTiling::intersection_point (:170-208) of the step segment A -> A + (dx, dy) with each
 // border of the unit square, the border's constant edge vector folded in: for C = (0,0),
-// D = (0,1), det = (bx-ax)*1 - (by-ay)*0 = bx-ax exactly, and so on. Every rounding happens in the
-// reference's operation order, so the results equal the general formula's bit for bit
+// D = (0,1), det = (bx-ax)*1 - (by-ay)*0 = bx-ax exactly, and so on (left / right: det = dx,
+// up / down: det = -dy). The border is crossed when t = nt / det and s = ns / det both lie in
+// [0, 1], tested without dividing (quotient_in_unit). Every rounding happens in the reference's
+// operation order, so the results equal the general formula's bit for bit
 // (tests/test_device_math.py checks this against the oracle). is_point_on_segment (:156-168)
-// reduces to "A lies on this edge"; |det| < 1e-9 rejects the border, as there.
-__host__ __device__ inline bool cross_left(double ax, double ay, double dx, double dy, double& ox, double& oy) {
-  if (ax == 0.0 && ay >= 0.0 && ay <= 1.0) return false;  // start on this border
-  const double det = dx;
-  if (fabs(det) < 1e-9) return false;
-  const double nt = -ax, ns = (-ax) * dy - (-ay) * dx;
-  if (!quotient_in_unit(nt, det) || !quotient_in_unit(ns, det)) return false;
-  const double t = nt / det;
-  ox = ax + t * dx;
-  oy = ay + t * dy;
-  return true;
-}
-__host__ __device__ inline bool cross_right(double ax, double ay, double dx, double dy, double& ox, double& oy) {
-  if (ax == 1.0 && ay >= 0.0 && ay <= 1.0) return false;
-  const double det = dx;
-  if (fabs(det) < 1e-9) return false;
-  const double cx = 1.0 - ax;
-  const double nt = cx, ns = cx * dy - (-ay) * dx;
-  if (!quotient_in_unit(nt, det) || !quotient_in_unit(ns, det)) return false;
-  const double t = nt / det;
-  ox = ax + t * dx;
-  oy = ay + t * dy;
-  return true;
-}
-__host__ __device__ inline bool cross_up(double ax, double ay, double dx, double dy, double& ox, double& oy) {
-  if (ay == 1.0 && ax >= 0.0 && ax <= 1.0) return false;
-  const double det = -dy;
-  if (fabs(det) < 1e-9) return false;
-  const double cy = 1.0 - ay;
-  const double nt = -cy, ns = (-ax) * dy - cy * dx;
-  if (!quotient_in_unit(nt, det) || !quotient_in_unit(ns, det)) return false;
-  const double t = nt / det;
-  ox = ax + t * dx;
-  oy = ay + t * dy;
-  return true;
-}
-__host__ __device__ inline bool cross_down(double ax, double ay, double dx, double dy, double& ox, double& oy) {
-  if (ay == 0.0 && ax >= 0.0 && ax <= 1.0) return false;
-  const double det = -dy;
-  if (fabs(det) < 1e-9) return false;
-  const double nt = ay, ns = (-ax) * dy - (-ay) * dx;
-  if (!quotient_in_unit(nt, det) || !quotient_in_unit(ns, det)) return false;
-  const double t = nt / det;
-  ox = ax + t * dx;
-  oy = ay + t * dy;
-  return true;
-}
+// reduces to "A lies on this edge", which rejects the border; so does |det| < 1e-9.
 
 // Seam wrap of one particle: the reference's restart loop (EuclideanTiling.cpp:31-69) acts on
 // each particle independently, so it runs here as a per-particle loop. (ax, ay) -> (px, py) is the
@@ -126,24 +85,38 @@ __host__ __device__ inline bool seam_wrap(double ax, double ay, double& px, doub
   while (!inside_unit(px, py)) {
     if (crossings >= max_crossings) { ok = false; break; }
     ++crossings;
-    // check_border_crossings (:71-101): left, right, up, down; exit snaps to 0 below 1e-3
-    double ex = ax, ey = ay;
-    int border = 4;
+    // check_border_crossings (:71-101): the first of left, right, up, down that the segment
+    // crosses; exit snaps to 0 below 1e-3. The four border tests (left, right, up, down) are
+    // evaluated side by side (no short circuit) so the loop's dependency chain holds one test,
+    // not four; only the chosen border's quotient is divided out.
     const double dx = px - ax, dy = py - ay;
-    if (cross_left(ax, ay, dx, dy, ex, ey)) border = 0;
-    else if (cross_right(ax, ay, dx, dy, ex, ey)) border = 1;
-    else if (cross_up(ax, ay, dx, dy, ex, ey)) border = 2;
-    else if (cross_down(ax, ay, dx, dy, ex, ey)) border = 3;
+    const double nl = -ax, nr = 1.0 - ax, nu = -(1.0 - ay), nd = ay;
+    const double sl = (-ax) * dy - (-ay) * dx, sr = (1.0 - ax) * dy - (-ay) * dx;
+    const double su = (-ax) * dy - (1.0 - ay) * dx, sd = (-ax) * dy - (-ay) * dx;
+    const bool on_lr = ay >= 0.0 && ay <= 1.0, on_ud = ax >= 0.0 && ax <= 1.0;
+    const bool okx = !(fabs(dx) < 1e-9), oky = !(fabs(-dy) < 1e-9);
+    const bool cl = all_of4(!(ax == 0.0 && on_lr), okx, quotient_in_unit(nl, dx), quotient_in_unit(sl, dx));
+    const bool cr = all_of4(!(ax == 1.0 && on_lr), okx, quotient_in_unit(nr, dx), quotient_in_unit(sr, dx));
+    const bool cu = all_of4(!(ay == 1.0 && on_ud), oky, quotient_in_unit(nu, -dy), quotient_in_unit(su, -dy));
+    const bool cd = all_of4(!(ay == 0.0 && on_ud), oky, quotient_in_unit(nd, -dy), quotient_in_unit(sd, -dy));
+    const int border = cl ? 0 : (cr ? 1 : (cu ? 2 : (cd ? 3 : 4)));
+    double ex = ax, ey = ay;
     if (border < 4) {
+      const double nt = border == 0 ? nl : (border == 1 ? nr : (border == 2 ? nu : nd));
+      const double det = border <= 1 ? dx : -dy;
+      const double t = nt / det;
+      ex = ax + t * dx;
+      ey = ay + t * dy;
       if (fabs(ex) < 1e-3) ex = 0.0;
       if (fabs(ey) < 1e-3) ey = 0.0;
     }
     // processPoints (:112-154)
     const double ox = px, oy = py;
-    if (border == 0) { px = oy; py = -ox; dtheta -= 0.5 * kPi; --quarter; }
-    else if (border == 1) { px = oy; py = 2 - ox; dtheta -= 0.5 * kPi; --quarter; }
-    else if (border == 2) { px = 2 - oy; py = ox; dtheta -= 1.5 * kPi; ++quarter; }
-    else { px = -oy; py = ox; dtheta -= 1.5 * kPi; ++quarter; }
+    const bool side = border <= 1;  // left / right: rotate by -90 degrees, else by +90
+    px = side ? oy : (border == 2 ? 2 - oy : -oy);
+    py = border == 0 ? -ox : (border == 1 ? 2 - ox : ox);
+    dtheta -= side ? 0.5 * kPi : 1.5 * kPi;
+    quarter += side ? -1 : 1;
     ax = ey;  // entry point = swapped exit point
     ay = ex;
   }
