@@ -172,6 +172,7 @@ __device__ __forceinline__ StepOut particle_step(const MeshDev& mesh, const Step
                                         float th, int v, float eta, int lane) {
   float Fx = 0.f, Fy = 0.f, Sx = 0.f, Sy = 0.f;
   double Fx64 = 0.0, Fy64 = 0.0;  // the same force in FP64 (used when the displacement is large)
+  const double kc64 = prm.k64 / prm.two_sigma64;  // Fij / d = kc (d - 2 sigma) / d, to a few ulps
   int na = 0, nb = 0;
   const int rs = mesh.row_ptr[v], re = mesh.row_ptr[v + 1];
   // Each lane walks its entries rs + lane + m * LANES in order, kRowChunk at a time: the chunk's
@@ -199,7 +200,7 @@ __device__ __forceinline__ StepOut particle_step(const MeshDev& mesh, const Step
       }
       const float g = pair_coef(d, prm);
       const double dd = d == 0.0f ? 0.001 : (double)d;
-      const double g64 = ((-prm.k64 * (prm.two_sigma64 - dd)) / prm.two_sigma64) / dd;
+      const double g64 = (kc64 * (dd - prm.two_sigma64)) / dd;
       const bool counted = d != 0.0f;
       src.each(ent[k].x, head[k], [&](int j) {
         const float4 q = src.pos(j);

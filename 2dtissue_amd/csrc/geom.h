@@ -36,7 +36,12 @@ struct MeshDev {
 
 // ------------------------------------------------------------------ seam wrap (FP64)
 
-__host__ __device__ inline bool inside_unit(double x, double y) { return x >= 0.0 && x <= 1.0 && y >= 0.0 && y <= 1.0; }
+// The predicates below combine their comparisons with & and | rather than && and ||: the
+// comparisons have no side effects, and without short-circuit sequencing the compiler evaluates
+// them side by side instead of branching on each one.
+__host__ __device__ inline bool inside_unit(double x, double y) {
+  return ((int)(x >= 0.0) & (int)(x <= 1.0) & (int)(y >= 0.0) & (int)(y <= 1.0)) != 0;
+}
 
 // ascending sort of 8 values in registers: Batcher's odd-even merge network (19 comparators,
 // fixed indices, so the values stay in registers)
@@ -54,7 +59,7 @@ __host__ __device__ inline void sort8(int (&a)[8]) {
 
 // fl(n / d) in [0, 1], decided without dividing (d != 0)
 __host__ __device__ inline bool quotient_in_unit(double n, double d) {
-  return (n == 0.0 || ((n > 0.0) == (d > 0.0))) && fabs(n) <= fabs(d);
+  return (((int)(n == 0.0) | (int)((n > 0.0) == (d > 0.0))) & (int)(fabs(n) <= fabs(d))) != 0;
 }
 
 // conjunction without short circuit: every operand is evaluated
@@ -93,22 +98,23 @@ __host__ __device__ inline bool seam_wrap(double ax, double ay, double& px, doub
     const double nl = -ax, nr = 1.0 - ax, nu = -(1.0 - ay), nd = ay;
     const double sl = (-ax) * dy - (-ay) * dx, sr = (1.0 - ax) * dy - (-ay) * dx;
     const double su = (-ax) * dy - (1.0 - ay) * dx, sd = (-ax) * dy - (-ay) * dx;
-    const bool on_lr = ay >= 0.0 && ay <= 1.0, on_ud = ax >= 0.0 && ax <= 1.0;
-    const bool okx = !(fabs(dx) < 1e-9), oky = !(fabs(-dy) < 1e-9);
-    const bool cl = all_of4(!(ax == 0.0 && on_lr), okx, quotient_in_unit(nl, dx), quotient_in_unit(sl, dx));
-    const bool cr = all_of4(!(ax == 1.0 && on_lr), okx, quotient_in_unit(nr, dx), quotient_in_unit(sr, dx));
-    const bool cu = all_of4(!(ay == 1.0 && on_ud), oky, quotient_in_unit(nu, -dy), quotient_in_unit(su, -dy));
-    const bool cd = all_of4(!(ay == 0.0 && on_ud), oky, quotient_in_unit(nd, -dy), quotient_in_unit(sd, -dy));
+    const int on_lr = (int)(ay >= 0.0) & (int)(ay <= 1.0), on_ud = (int)(ax >= 0.0) & (int)(ax <= 1.0);
+    const int okx = !(fabs(dx) < 1e-9), oky = !(fabs(-dy) < 1e-9);
+    const bool cl = all_of4(!((int)(ax == 0.0) & on_lr), okx, quotient_in_unit(nl, dx), quotient_in_unit(sl, dx));
+    const bool cr = all_of4(!((int)(ax == 1.0) & on_lr), okx, quotient_in_unit(nr, dx), quotient_in_unit(sr, dx));
+    const bool cu = all_of4(!((int)(ay == 1.0) & on_ud), oky, quotient_in_unit(nu, -dy), quotient_in_unit(su, -dy));
+    const bool cd = all_of4(!((int)(ay == 0.0) & on_ud), oky, quotient_in_unit(nd, -dy), quotient_in_unit(sd, -dy));
     const int border = cl ? 0 : (cr ? 1 : (cu ? 2 : (cd ? 3 : 4)));
-    double ex = ax, ey = ay;
-    if (border < 4) {
-      const double nt = border == 0 ? nl : (border == 1 ? nr : (border == 2 ? nu : nd));
-      const double det = border <= 1 ? dx : -dy;
-      const double t = nt / det;
-      ex = ax + t * dx;
-      ey = ay + t * dy;
-      if (fabs(ex) < 1e-3) ex = 0.0;
-      if (fabs(ey) < 1e-3) ey = 0.0;
+    // exit point of the chosen border (border 4, no border crossed: the exit point stays A)
+    const double nt = border == 0 ? nl : (border == 1 ? nr : (border == 2 ? nu : nd));
+    const double det = border <= 1 ? dx : (border < 4 ? -dy : 1.0);
+    const double t = nt / det;
+    double ex = ax + t * dx, ey = ay + t * dy;
+    if (fabs(ex) < 1e-3) ex = 0.0;
+    if (fabs(ey) < 1e-3) ey = 0.0;
+    if (border == 4) {
+      ex = ax;
+      ey = ay;
     }
     // processPoints (:112-154)
     const double ox = px, oy = py;
