@@ -136,7 +136,7 @@ struct tissue_mesh {
   float* d_xyz = nullptr;
   int32_t* d_faces = nullptr;
   int32_t* d_cell_start = nullptr;
-  int32_t* d_cell_faces = nullptr;
+  float4* d_cell_tri = nullptr;
   int32_t* d_row_ptr = nullptr;
   int2* d_nbr = nullptr;
 
@@ -145,7 +145,7 @@ struct tissue_mesh {
     int cur = 0;
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(device);
-    dfree(d_uv); dfree(d_xyz); dfree(d_faces); dfree(d_cell_start); dfree(d_cell_faces); dfree(d_row_ptr); dfree(d_nbr);
+    dfree(d_uv); dfree(d_xyz); dfree(d_faces); dfree(d_cell_start); dfree(d_cell_tri); dfree(d_row_ptr); dfree(d_nbr);
     (void)hipSetDevice(cur);
     dev_ready = false;
   }
@@ -172,14 +172,17 @@ struct tissue_mesh {
       d_xyz = dalloc<float>(xyz.size()); upload(d_xyz, xyz.data(), xyz.size());
       d_faces = dalloc<int32_t>(m.faces.size()); upload(d_faces, m.faces.data(), m.faces.size());
       d_cell_start = dalloc<int32_t>(m.cell_start.size()); upload(d_cell_start, m.cell_start.data(), m.cell_start.size());
-      d_cell_faces = dalloc<int32_t>(m.cell_faces.size()); upload(d_cell_faces, m.cell_faces.data(), m.cell_faces.size());
+      std::vector<float4> tri(2 * m.cell_faces.size());
+      for (size_t q = 0; q < m.cell_faces.size(); ++q)
+        tdev::pack_cell_triangle(reinterpret_cast<const float*>(uv.data()), m.faces.data(), m.cell_faces[q], &tri[2 * q]);
+      d_cell_tri = dalloc<float4>(tri.size()); upload(d_cell_tri, tri.data(), tri.size());
       d_row_ptr = dalloc<int32_t>(m.row_ptr.size()); upload(d_row_ptr, m.row_ptr.data(), m.row_ptr.size());
       d_nbr = dalloc<int2>(nbr.size()); upload(d_nbr, nbr.data(), nbr.size());
       dev_ready = true;
     }
     tdev::MeshDev v;
     v.nv = m.nv; v.nf = m.nf; v.grid = m.grid; v.pad = 0;
-    v.uv = d_uv; v.xyz = d_xyz; v.faces = d_faces; v.cell_start = d_cell_start; v.cell_faces = d_cell_faces;
+    v.uv = d_uv; v.xyz = d_xyz; v.faces = d_faces; v.cell_start = d_cell_start; v.cell_tri = d_cell_tri;
     v.row_ptr = d_row_ptr; v.nbr = d_nbr;
     return v;
   }

@@ -29,7 +29,7 @@ struct MeshDev {
   const float* xyz;        // [nv*3]
   const int32_t* faces;    // [nf*3]
   const int32_t* cell_start;
-  const int32_t* cell_faces;
+  const float4* cell_tri;  // [2 * n_cell_faces]: candidate faces of each cell with their UV corners
   const int32_t* row_ptr;  // [nv+1]
   const int2* nbr;         // (vertex, float bits of the geodesic distance), rows ascending in distance
 };
@@ -181,6 +181,75 @@ __host__ __device__ inline T point_triangle_distance(T px, T py, T ax, T ay, T b
   return sqrt(qx * qx + qy * qy);
 }
 
+// Candidate record of a face bucket: entry q of the cell's face list as two float4,
+// (ax, ay, bx, by) and (cx, cy, face id bits, 0), so one load level yields a candidate's corners.
+inline void pack_cell_triangle(const float* uv, const int32_t* faces, int32_t f, float4* out) {
+  const int32_t a = faces[3 * f], b = faces[3 * f + 1], c = faces[3 * f + 2];
+  float fbits;
+  static_assert(sizeof(fbits) == sizeof(f), "face id bits");
+  __builtin_memcpy(&fbits, &f, sizeof f);
+  out[0] = make_float4(uv[2 * a], uv[2 * a + 1], uv[2 * b], uv[2 * b + 1]);
+  out[1] = make_float4(uv[2 * c], uv[2 * c + 1], fbits, 0.0f);
+}
+
+__host__ __device__ inline int float_bits(float x) {
+  int b;
+  __builtin_memcpy(&b, &x, sizeof b);
+  return b;
+}
+
+__host__ __device__ inline float approx_rcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(x);  // v_rcp_f32, about 1 ulp: enough for the FP32 ranking
+#else
+  return 1.0f / x;
+#endif
+}
+
+// Squared distance from P to triangle ABC (z = 0) for the FP32 ranking: the closest point is
+// chosen by the tests of point_triangle_distance in the same order (vertex A, B, C, edge AB, AC,
+// BC, interior), evaluated side by side and combined with selects, so lanes whose candidates fall
+// in different regions do not diverge. A vertex region is the segment from that vertex with
+// t = 0. Same mathematics as point_triangle_distance; reciprocals are approximate (the ranking
+// is only kept when the best two distances differ by kLocateMargin).
+__host__ __device__ inline float point_triangle_dist2(float px, float py, float ax, float ay, float bx, float by,
+                                                      float cx, float cy) {
+  const float abx = bx - ax, aby = by - ay, acx = cx - ax, acy = cy - ay;
+  const float apx = px - ax, apy = py - ay, bpx = px - bx, bpy = py - by, cpx = px - cx, cpy = py - cy;
+  const float d_ab_ap = abx * apx + aby * apy, d_ac_ap = acx * apx + acy * apy;
+  const float d_ab_bp = abx * bpx + aby * bpy, d_ac_bp = acx * bpx + acy * bpy;
+  const float d_ab_cp = abx * cpx + aby * cpy, d_ac_cp = acx * cpx + acy * cpy;
+  const float vc = d_ab_ap * d_ac_bp - d_ab_bp * d_ac_ap;
+  const float vb = d_ab_cp * d_ac_ap - d_ab_ap * d_ac_cp;
+  const float va = d_ab_bp * d_ac_cp - d_ab_cp * d_ac_bp;
+  const int in_a = (int)(d_ab_ap <= 0.0f) & (int)(d_ac_ap <= 0.0f);
+  const int in_b = (int)(d_ab_bp >= 0.0f) & (int)(d_ac_bp <= d_ab_bp);
+  const int in_c = (int)(d_ac_cp >= 0.0f) & (int)(d_ab_cp <= d_ac_cp);
+  const int on_ab = (int)(vc <= 0.0f) & (int)(d_ab_ap >= 0.0f) & (int)(d_ab_bp <= 0.0f);
+  const int on_ac = (int)(vb <= 0.0f) & (int)(d_ac_ap >= 0.0f) & (int)(d_ac_cp <= 0.0f);
+  const int on_bc = (int)(va <= 0.0f) & (int)((d_ac_bp - d_ab_bp) >= 0.0f) & (int)((d_ab_cp - d_ac_cp) >= 0.0f);
+  const int region = in_a ? 0 : (in_b ? 1 : (in_c ? 2 : (on_ab ? 3 : (on_ac ? 4 : (on_bc ? 5 : 6)))));
+  // segment S0 -> S1 of the vertex / edge regions
+  const bool from_a = region == 0 || region == 3 || region == 4, from_b = region == 1 || region == 5;
+  const float s0x = from_a ? ax : (from_b ? bx : cx), s0y = from_a ? ay : (from_b ? by : cy);
+  const float s1x = region == 3 ? bx : cx, s1y = region == 3 ? by : cy;
+  const bool vertex = region <= 2;
+  const float ex = s1x - s0x, ey = s1y - s0y;
+  const float num = vertex ? 0.0f : ex * (px - s0x) + ey * (py - s0y);
+  const float den = vertex ? 1.0f : ex * ex + ey * ey;
+  float t = num * approx_rcp(den);
+  t = t < 0.0f ? 0.0f : (1.0f < t ? 1.0f : t);
+  float qx = s0x + ex * t - px, qy = s0y + ey * t - py;
+  // interior: barycentric closest point
+  const float denom = approx_rcp(va + vb + vc);
+  const float v = vb * denom, w = vc * denom;
+  if (region == 6) {
+    qx = ax + abx * v + acx * w - px;
+    qy = ay + aby * v + acy * w - py;
+  }
+  return qx * qx + qy * qy;
+}
+
 struct Located {
   int face, vid;
   double X, Y, Z;
@@ -226,29 +295,22 @@ __host__ __device__ inline Located locate(const MeshDev& m, double px, double py
   r.fp64 = 0;
   // FP32 ranking: best and runner-up distance
   const float qx = (float)px, qy = (float)py;
-  float b1 = __builtin_inff(), b2 = __builtin_inff();
+  float b1 = __builtin_inff(), b2 = __builtin_inff();  // squared distances
   int f1 = 0x7fffffff;
-  // candidates in chunks of kFaceChunk per lane: face ids, then their corners, then the corners'
-  // UV coordinates are loaded together (same visiting order as one at a time)
+  // candidates in chunks of kFaceChunk per lane, their records loaded together (same visiting
+  // order as one at a time)
   for (int q0 = cs + lane; q0 < ce; q0 += kFaceChunk * LANES) {
-    int f[kFaceChunk], ia[kFaceChunk], ib[kFaceChunk], ic[kFaceChunk];
-    float2 a[kFaceChunk], b[kFaceChunk], c[kFaceChunk];
+    float4 e0[kFaceChunk], e1[kFaceChunk];
 #pragma unroll
     for (int k = 0; k < kFaceChunk; ++k) {
       const int q = q0 + k * LANES;
-      f[k] = q < ce ? m.cell_faces[q] : -1;
+      if (q < ce) { e0[k] = m.cell_tri[2 * q]; e1[k] = m.cell_tri[2 * q + 1]; }
     }
 #pragma unroll
     for (int k = 0; k < kFaceChunk; ++k)
-      if (f[k] >= 0) { ia[k] = m.faces[3 * f[k]]; ib[k] = m.faces[3 * f[k] + 1]; ic[k] = m.faces[3 * f[k] + 2]; }
-#pragma unroll
-    for (int k = 0; k < kFaceChunk; ++k)
-      if (f[k] >= 0) { a[k] = m.uv[ia[k]]; b[k] = m.uv[ib[k]]; c[k] = m.uv[ic[k]]; }
-#pragma unroll
-    for (int k = 0; k < kFaceChunk; ++k)
-      if (f[k] >= 0) {
-        const float d = point_triangle_distance<float>(qx, qy, a[k].x, a[k].y, b[k].x, b[k].y, c[k].x, c[k].y);
-        merge_best(b1, f1, b2, d, f[k], __builtin_inff());
+      if (q0 + k * LANES < ce) {
+        const float d2 = point_triangle_dist2(qx, qy, e0[k].x, e0[k].y, e0[k].z, e0[k].w, e1[k].x, e1[k].y);
+        merge_best(b1, f1, b2, d2, float_bits(e1[k].z), __builtin_inff());
       }
   }
   if constexpr (LANES > 1) {
@@ -260,14 +322,14 @@ __host__ __device__ inline Located locate(const MeshDev& m, double px, double py
     }
   }
   int best_f = f1;
-  if (!(b2 - b1 > kLocateMargin)) {  // close call (or non-finite input): rank in FP64
+  if (!(sqrtf(b2) - sqrtf(b1) > kLocateMargin)) {  // close call (or non-finite input): rank in FP64
     r.fp64 |= 1;
     double bd = __builtin_inf();
     int bf = 0x7fffffff;
     for (int q = cs + lane; q < ce; q += LANES) {
-      const int f = m.cell_faces[q];
-      const float2 a = m.uv[m.faces[3 * f]], b = m.uv[m.faces[3 * f + 1]], c = m.uv[m.faces[3 * f + 2]];
-      const double d = point_triangle_distance<double>(px, py, a.x, a.y, b.x, b.y, c.x, c.y);
+      const float4 e0 = m.cell_tri[2 * q], e1 = m.cell_tri[2 * q + 1];
+      const int f = float_bits(e1.z);
+      const double d = point_triangle_distance<double>(px, py, e0.x, e0.y, e0.z, e0.w, e1.x, e1.y);
       if (lex_less(d, f, bd, bf)) { bd = d; bf = f; }
     }
     if constexpr (LANES > 1) {

@@ -1,6 +1,8 @@
 // Host build of the device geometry (2dtissue_amd/csrc/geom.h) for exactness tests against the
 // FP64 oracle. Test infrastructure: compiled by tests/hostcheck/Makefile, loaded by
 // tests/test_device_math.py; nothing in the library uses it.
+#include <vector>
+
 #include "../../2dtissue_amd/csrc/geom.h"
 
 extern "C" {
@@ -29,6 +31,16 @@ void hc_seam_wrap_many(int n, const double* a, const double* p, int max_crossing
   }
 }
 
+// FP32 branch-free squared distance and the FP64 region-by-region distance, for n (point,
+// triangle) pairs: tri = (ax, ay, bx, by, cx, cy) per pair
+void hc_tri_dist_many(int n, const float* pts, const float* tri, float* d2_f32, double* d_f64) {
+  for (int i = 0; i < n; ++i) {
+    const float* t = tri + 6 * i;
+    d2_f32[i] = tdev::point_triangle_dist2(pts[2 * i], pts[2 * i + 1], t[0], t[1], t[2], t[3], t[4], t[5]);
+    d_f64[i] = tdev::point_triangle_distance<double>(pts[2 * i], pts[2 * i + 1], t[0], t[1], t[2], t[3], t[4], t[5]);
+  }
+}
+
 void hc_locate_many(int n, const double* pts, int nv, int nf, const float* uv, const float* xyz, const int* faces,
                     int grid, const int* cell_start, const int* cell_faces, int* face, int* vid, double* r3d,
                     int* fp64) {
@@ -40,7 +52,9 @@ void hc_locate_many(int n, const double* pts, int nv, int nf, const float* uv, c
   m.xyz = xyz;
   m.faces = faces;
   m.cell_start = cell_start;
-  m.cell_faces = cell_faces;
+  std::vector<float4> tri(2 * (size_t)cell_start[grid * grid]);
+  for (int q = 0; q < cell_start[grid * grid]; ++q) tdev::pack_cell_triangle(uv, faces, cell_faces[q], &tri[2 * (size_t)q]);
+  m.cell_tri = tri.data();
   for (int i = 0; i < n; ++i) {
     const tdev::Located L = tdev::locate<1>(m, pts[2 * i], pts[2 * i + 1], 0);
     face[i] = L.face;

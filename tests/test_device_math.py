@@ -148,6 +148,24 @@ def test_locate_matches_oracle_near_edges_and_vertices(H, O, mesh32):
     assert (fp & 1).any() and (fp & 2).any()  # the FP64 fallbacks are exercised
 
 
+def test_triangle_distance_branch_free(H):
+    """The FP32 ranking's branch-free squared distance against the region-by-region FP64 distance
+    (the reference's case analysis), over points in every Voronoi region of random triangles."""
+    rng = np.random.default_rng(9)
+    n = 200000
+    c = rng.random((n, 2))
+    tri = (c[:, None, :] + rng.normal(0, 0.01, (n, 3, 2))).reshape(n, 6)
+    pts = c + rng.normal(0, 0.02, (n, 2))
+    tri32 = np.ascontiguousarray(tri, np.float32)
+    pts32 = np.ascontiguousarray(pts, np.float32)
+    d2 = np.zeros(n, np.float32)
+    d64 = np.zeros(n)
+    H.hc_tri_dist_many(n, p(pts32, C.c_float), p(tri32, C.c_float), p(d2, C.c_float), p(d64, C.c_double))
+    err = np.abs(np.sqrt(d2.astype(np.float64)) - d64)
+    assert err.max() < 1e-6, err.max()  # far below the ranking margin (1e-5)
+    assert (d64 == 0).mean() > 0.02 and (d64 > 0).mean() > 0.5  # interior and exterior points
+
+
 def test_sort8_network(H):
     """The bucket sort network sorts every 0-1 input of length 8 (0-1 principle: then it sorts
     every input), and random and padded (INT_MAX) inputs as the buckets use it."""
