@@ -5,9 +5,11 @@ The package directory starts with a digit, so import it with
 ``make -C 2dtissue_amd`` (``__graft_entry__.build()``); loading fails loudly when it is missing:
 there is no CPU fallback.
 
-Reference interface mirrored (KAIZN-io/2DTissue, src/simulation/2DTissue.h:34-129):
-``Simulation.start`` / ``update`` / ``is_finished`` / ``get_order_parameter`` wrap the C-ABI the
-same way include/2DTissue.h does for C++.
+Classes: ``Mesh`` (surface, UV chart and tables), ``Sim`` (one system: the per-step update of
+KAIZN-io/2DTissue src/simulation/2DTissue.cpp:136-206 and its outputs), ``Batch`` (replicas with
+their own sizes and parameters in one launch sequence), plus ``loss`` / ``Sim.adjoint`` for
+gradients. ``SIGNATURES`` lists every C-ABI entry point with its ctypes argument types. The C++
+drop-in of the reference's ``_2DTissue`` class is the header include/2DTissue.h.
 """
 from __future__ import annotations
 
