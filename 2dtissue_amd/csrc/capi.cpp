@@ -174,6 +174,44 @@ int32_t tissue_gen_noise(uint64_t seed, double std_dev, int32_t n_steps, int32_t
   });
 }
 
+int32_t tissue_seam_wrap(int32_t n, const double* start_xy, double* end_xy, double* dtheta, int32_t* quarter,
+                         int32_t max_crossings, int32_t lanes) {
+  return guarded([&] {
+    if (n < 0 || (n > 0 && (!start_xy || !end_xy || !dtheta))) throw ApiError(TISSUE_E_INVALID_ARG, "bad seam-wrap arguments");
+    if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16)
+      throw ApiError(TISSUE_E_INVALID_ARG, "lanes must be 1, 2, 4, 8 or 16");
+    if ((int64_t)n * lanes > INT_MAX - 256) throw ApiError(TISSUE_E_INVALID_ARG, "too many moves for one call");
+    for (int64_t q = 0; q < 2 * (int64_t)n; ++q)
+      if (!std::isfinite(start_xy[q]) || !std::isfinite(end_xy[q])) throw ApiError(TISSUE_E_INVALID_ARG, "non-finite position");
+    if (n == 0) return TISSUE_OK;
+    struct Buffers {
+      double2 *s = nullptr, *e = nullptr;
+      double* d = nullptr;
+      int *q = nullptr, *ok = nullptr;
+      ~Buffers() { dfree(s); dfree(e); dfree(d); dfree(q); dfree(ok); }
+    } b;
+    b.s = dalloc<double2>(n);
+    b.e = dalloc<double2>(n);
+    b.d = dalloc<double>(n);
+    b.q = dalloc<int>(n);
+    b.ok = dalloc<int>(n);
+    HIP_OK(hipMemcpy(b.s, start_xy, 2 * sizeof(double) * (size_t)n, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(b.e, end_xy, 2 * sizeof(double) * (size_t)n, hipMemcpyHostToDevice));
+    const int mc = max_crossings > 0 ? max_crossings : kDefaultMaxCrossings;
+    HIP_OK(tissue::launch_seam_wrap(lanes, n, b.s, b.e, b.d, b.q, b.ok, mc, nullptr));
+    HIP_OK(hipStreamSynchronize(nullptr));
+    std::vector<int> ok((size_t)n), qv((size_t)n);
+    HIP_OK(hipMemcpy(end_xy, b.e, 2 * sizeof(double) * (size_t)n, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(dtheta, b.d, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(qv.data(), b.q, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(ok.data(), b.ok, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost));
+    if (quarter) std::copy(qv.begin(), qv.end(), quarter);
+    for (int v : ok)
+      if (!v) return TISSUE_E_TILING_NO_CONVERGE;
+    return TISSUE_OK;
+  });
+}
+
 void tissue_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
   tissue::philox4x32_10(ctr, key, out);
 }

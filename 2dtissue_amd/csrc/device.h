@@ -310,7 +310,7 @@ __device__ __forceinline__ StepOut particle_step(const MeshDev& mesh, const Step
   double px = xn, py = yn, dth = 0.0;
   const bool finite = isfinite(xn) && isfinite(yn) && isfinite(thn);
   if (!finite) o.fl |= kFlagNonfinite;
-  else if (!seam_wrap(me.x, me.y, px, py, dth, prm.max_crossings, nullptr, &o.q)) o.fl |= kFlagTiling;
+  else if (!seam_wrap_group<LANES>(me.x, me.y, px, py, dth, prm.max_crossings, lane, &o.q)) o.fl |= kFlagTiling;
   if (finite && !inside_unit(px, py)) o.fl |= kFlagLost;
   if (finite) thn = wrap_heading(thn + dth);
   o.xo = (float)px;

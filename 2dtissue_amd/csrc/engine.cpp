@@ -379,8 +379,11 @@ Shape Engine::resolve_for(Shape s) const {
     s.xcd = 0;
   } else {
     // measured on MI355X: a single system of a few thousand particles fills few CUs, so more lanes
-    // per particle shorten its critical path (2,000 particles: 16 lanes 30.8 us, 8 lanes 33.8 us)
-    s.lanes = s.lanes > 0 ? s.lanes : (ntot <= 4096 ? 16 : (ntot <= 8192 ? 8 : (ntot <= 65536 ? 4 : 2)));
+    // per particle shorten its critical path (2,000 particles: 16 lanes 25.2 us, 8 lanes 27.2 us);
+    // a single 100,000-particle system is fastest with 4 (350 us; 2 lanes 383 us), a large batch
+    // with 2 (autotuned)
+    const int big_lanes = R == 1 ? 4 : 2;
+    s.lanes = s.lanes > 0 ? s.lanes : (ntot <= 4096 ? 16 : (ntot <= 8192 ? 8 : (ntot <= 65536 ? 4 : big_lanes)));
     s.block = s.block > 0 ? s.block : (ntot <= 4096 ? 128 : 256);
     s.spl = s.spl > 0 ? s.spl : kDefaultSpl;
     s.xcd = s.xcd >= 0 ? s.xcd : (R >= 2 * kXcds ? 1 : 0);

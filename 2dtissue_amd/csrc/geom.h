@@ -57,22 +57,111 @@ __host__ __device__ inline void sort8(int (&a)[8]) {
   }
 }
 
-// fl(n / d) in [0, 1], decided without dividing (d != 0)
+__host__ __device__ inline uint64_t bits64(double x) {
+  uint64_t b;
+  __builtin_memcpy(&b, &x, sizeof b);
+  return b;
+}
+__host__ __device__ inline double from_bits64(uint64_t b) {
+  double x;
+  __builtin_memcpy(&x, &b, sizeof x);
+  return x;
+}
+constexpr uint64_t kSignBit = 0x8000000000000000ull;
+// x with its sign bit xor-ed with m (an exact negation when m = kSignBit)
+__host__ __device__ inline double flip(double x, uint64_t m) { return from_bits64(bits64(x) ^ m); }
+
+// fl(n / d) in [0, 1] given nf = n with the sign of d folded in (n negated when d's sign bit is
+// set) and ad = |d|: fl(n / d) >= 0 exactly when nf >= 0 (zero of either sign included), and for
+// nf >= 0, fl(nf / ad) <= 1 exactly when nf <= ad (rounding is monotone and 1 is representable);
+// NaN fails both ways, as the quotient would.
+__host__ __device__ inline int folded_in_unit(double nf, double ad) { return (int)(nf >= 0.0) & (int)(nf <= ad); }
+
+// fl(n / d) in [0, 1], decided without dividing
 __host__ __device__ inline bool quotient_in_unit(double n, double d) {
-  return (((int)(n == 0.0) | (int)((n > 0.0) == (d > 0.0))) & (int)(fabs(n) <= fabs(d))) != 0;
+  return folded_in_unit(flip(n, bits64(d) & kSignBit), fabs(d)) != 0;
 }
 
-// conjunction without short circuit: every operand is evaluated
-__host__ __device__ inline bool all_of4(bool a, bool b, bool c, bool d) { return ((int)a & (int)b & (int)c & (int)d) != 0; }
+// EuclideanTiling::intersection_point (:170-208) of the step segment A -> A + (dx, dy) with
+// border k of the unit square (0 left, 1 right, 2 up, 3 down), the border's constant edge vector
+// folded in: for the left border C = (0,0), D = (0,1), det = (bx-ax)*1 - (by-ay)*0 = bx-ax
+// exactly, and so on (left / right: det = dx, up / down: det = -dy). The border is crossed when
+// t = n / det and s = (a dy - b dx) / det both lie in [0, 1]: left n = -ax, right 1 - ax, up
+// -(1 - ay), down ay; a = 1 - ax for the right border, -ax otherwise; b = 1 - ay for the upper
+// border, -ay otherwise. Every rounding happens in the reference's operation order, so the tests
+// decide as the general formula does (tests/test_device_math.py checks the loop against the
+// oracle). is_point_on_segment (:156-168) reduces to "A lies on this edge", which rejects the
+// border; so does |det| < 1e-9. omx = 1 - ax, omy = 1 - ay. With k a compile-time constant the
+// selects fold away.
+__host__ __device__ inline int border_test(int k, double ax, double ay, double dx, double dy, double omx, double omy) {
+  const bool lr = k < 2;
+  const double u = lr ? ax : ay, w = lr ? ay : ax;  // coordinate across / along the edge
+  const double n = flip((k == 1 || k == 2) ? (lr ? omx : omy) : u, (k == 0 || k == 2) ? kSignBit : 0ull);
+  const double det = lr ? dx : -dy;
+  const double sv = (k == 1 ? omx : -ax) * dy - (k == 2 ? omy : -ay) * dx;
+  const uint64_t f = bits64(det) & kSignBit;
+  const double ad = fabs(det);
+  const int on_edge = (int)(u == ((k == 0 || k == 3) ? 0.0 : 1.0)) & (int)(w >= 0.0) & (int)(w <= 1.0);
+  return (on_edge ^ 1) & (int)!(ad < 1e-9) & folded_in_unit(flip(n, f), ad) & folded_in_unit(flip(sv, f), ad);
+}
 
-// EuclideanTiling::intersection_point (:170-208) of the step segment A -> A + (dx, dy) with each
-// border of the unit square, the border's constant edge vector folded in: for C = (0,0),
-// D = (0,1), det = (bx-ax)*1 - (by-ay)*0 = bx-ax exactly, and so on (left / right: det = dx,
-// up / down: det = -dy). The border is crossed when t = nt / det and s = ns / det both lie in
-// [0, 1], tested without dividing (quotient_in_unit). Every rounding happens in the reference's
-// operation order, so the results equal the general formula's bit for bit
-// (tests/test_device_math.py checks this against the oracle). is_point_on_segment (:156-168)
-// reduces to "A lies on this edge", which rejects the border; so does |det| < 1e-9.
+// 1 / d refined for quotient(): on the device the hardware reciprocal with two Newton-Raphson
+// steps, started as soon as d is known (before the border decision); unused on the host
+__host__ __device__ inline double quotient_reciprocal(double d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
+#else
+  return d;
+#endif
+}
+
+// fl(n / d), IEEE round to nearest. Device: q = n r, then one remainder correction
+// q + (n - d q) r (Markstein, IBM J. Res. Dev. 34(1), 1990), the same steps as the compiler's
+// division without its exponent pre-scaling and special-value fix-up, neither of which applies
+// to the operands here (|d| in [1e-9, 1e6], |n| <= 2, no zero or non-finite divisor).
+// tests/test_gpu_wrap.py checks the GPU loop against the oracle bit for bit.
+__host__ __device__ inline double quotient(double n, double d, double r) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double q = n * r;
+  const double rem = fma(-d, q, n);
+  return fma(rem, r, q);
+#else
+  (void)r;
+  return n / d;
+#endif
+}
+
+// One crossing once the border is known (border 4: none found): exit point by
+// intersection_point with the exit snapped to 0 below 1e-3 (check_border_crossings :71-101), the
+// end point mapped by processPoints (:112-154), the heading rotation, the quarter turns of the
+// composed map, and the entry point for the next test (the swapped exit point). rx, ry: the
+// quotient reciprocals of dx and -dy.
+__host__ __device__ inline void cross_border(int border, double& ax, double& ay, double& px, double& py, double dx,
+                                             double dy, double omx, double omy, double rx, double ry, double& dtheta,
+                                             int& quarter) {
+  const bool side = border <= 1;  // left / right: rotate by -90 degrees, else by +90
+  const double nt = side ? (border == 0 ? -ax : omx) : (border == 2 ? -omy : ay);
+  const double det = side ? dx : (border < 4 ? -dy : 1.0);
+  const double t = quotient(nt, det, side ? rx : (border < 4 ? ry : 1.0));
+  double ex = ax + t * dx, ey = ay + t * dy;
+  if (fabs(ex) < 1e-3) ex = 0.0;
+  if (fabs(ey) < 1e-3) ey = 0.0;
+  if (border == 4) {
+    ex = ax;
+    ey = ay;
+  }
+  const double ox = px, oy = py;
+  px = side ? oy : (border == 2 ? 2 - oy : -oy);
+  py = border == 0 ? -ox : (border == 1 ? 2 - ox : ox);
+  dtheta -= side ? 0.5 * kPi : 1.5 * kPi;
+  quarter += side ? -1 : 1;
+  ax = ey;  // entry point = swapped exit point
+  ay = ex;
+}
 
 // Seam wrap of one particle: the reference's restart loop (EuclideanTiling.cpp:31-69) acts on
 // each particle independently, so it runs here as a per-particle loop. (ax, ay) -> (px, py) is the
@@ -80,55 +169,66 @@ __host__ __device__ inline bool all_of4(bool a, bool b, bool c, bool d) { return
 // and dtheta holds the heading rotation (-pi/2 per left/right, -3 pi/2 per up/down crossing).
 // The composed map is p -> R p + b with R a rotation by quarter_out * 90 degrees
 // (counter-clockwise, 0..3); the adjoint needs only R. Returns false when more than
-// max_crossings crossings would be needed.
+// max_crossings crossings would be needed. check_border_crossings takes the first of left,
+// right, up, down that the segment crosses; the four tests are evaluated side by side.
 __host__ __device__ inline bool seam_wrap(double ax, double ay, double& px, double& py, double& dtheta,
                                           int max_crossings, int* crossings_out = nullptr, int* quarter_out = nullptr) {
   int crossings = 0;
-  int quarter = 0;  // counter-clockwise quarter turns of the composed map's linear part
+  int quarter = 0;
   dtheta = 0.0;
   bool ok = true;
   while (!inside_unit(px, py)) {
     if (crossings >= max_crossings) { ok = false; break; }
     ++crossings;
-    // check_border_crossings (:71-101): the first of left, right, up, down that the segment
-    // crosses; exit snaps to 0 below 1e-3. The four border tests (left, right, up, down) are
-    // evaluated side by side (no short circuit) so the loop's dependency chain holds one test,
-    // not four; only the chosen border's quotient is divided out.
-    const double dx = px - ax, dy = py - ay;
-    const double nl = -ax, nr = 1.0 - ax, nu = -(1.0 - ay), nd = ay;
-    const double sl = (-ax) * dy - (-ay) * dx, sr = (1.0 - ax) * dy - (-ay) * dx;
-    const double su = (-ax) * dy - (1.0 - ay) * dx, sd = (-ax) * dy - (-ay) * dx;
-    const int on_lr = (int)(ay >= 0.0) & (int)(ay <= 1.0), on_ud = (int)(ax >= 0.0) & (int)(ax <= 1.0);
-    const int okx = !(fabs(dx) < 1e-9), oky = !(fabs(-dy) < 1e-9);
-    const bool cl = all_of4(!((int)(ax == 0.0) & on_lr), okx, quotient_in_unit(nl, dx), quotient_in_unit(sl, dx));
-    const bool cr = all_of4(!((int)(ax == 1.0) & on_lr), okx, quotient_in_unit(nr, dx), quotient_in_unit(sr, dx));
-    const bool cu = all_of4(!((int)(ay == 1.0) & on_ud), oky, quotient_in_unit(nu, -dy), quotient_in_unit(su, -dy));
-    const bool cd = all_of4(!((int)(ay == 0.0) & on_ud), oky, quotient_in_unit(nd, -dy), quotient_in_unit(sd, -dy));
-    const int border = cl ? 0 : (cr ? 1 : (cu ? 2 : (cd ? 3 : 4)));
-    // exit point of the chosen border (border 4, no border crossed: the exit point stays A)
-    const double nt = border == 0 ? nl : (border == 1 ? nr : (border == 2 ? nu : nd));
-    const double det = border <= 1 ? dx : (border < 4 ? -dy : 1.0);
-    const double t = nt / det;
-    double ex = ax + t * dx, ey = ay + t * dy;
-    if (fabs(ex) < 1e-3) ex = 0.0;
-    if (fabs(ey) < 1e-3) ey = 0.0;
-    if (border == 4) {
-      ex = ax;
-      ey = ay;
-    }
-    // processPoints (:112-154)
-    const double ox = px, oy = py;
-    const bool side = border <= 1;  // left / right: rotate by -90 degrees, else by +90
-    px = side ? oy : (border == 2 ? 2 - oy : -oy);
-    py = border == 0 ? -ox : (border == 1 ? 2 - ox : ox);
-    dtheta -= side ? 0.5 * kPi : 1.5 * kPi;
-    quarter += side ? -1 : 1;
-    ax = ey;  // entry point = swapped exit point
-    ay = ex;
+    const double dx = px - ax, dy = py - ay, omx = 1.0 - ax, omy = 1.0 - ay;
+    const double rx = quotient_reciprocal(dx), ry = quotient_reciprocal(-dy);
+    const int bits = border_test(0, ax, ay, dx, dy, omx, omy) | (border_test(1, ax, ay, dx, dy, omx, omy) << 1) |
+                     (border_test(2, ax, ay, dx, dy, omx, omy) << 2) | (border_test(3, ax, ay, dx, dy, omx, omy) << 3);
+    const int border = (bits & 1) ? 0 : ((bits & 2) ? 1 : ((bits & 4) ? 2 : ((bits & 8) ? 3 : 4)));
+    cross_border(border, ax, ay, px, py, dx, dy, omx, omy, rx, ry, dtheta, quarter);
   }
   if (crossings_out) *crossings_out = crossings;
   if (quarter_out) *quarter_out = quarter & 3;
   return ok;
+}
+
+// The same loop run by the LANES lanes of one particle (a lane group aligned within the wave,
+// all of its lanes active): the four border tests are spread over the lanes (lane % 4 tests
+// border lane % 4; with two lanes, each lane tests two borders) and the first passing border is
+// read from a wave ballot, so each lane evaluates one or two tests instead of four. Same tests,
+// same arithmetic, same decisions as seam_wrap.
+template <int LANES>
+__device__ inline bool seam_wrap_group(double ax, double ay, double& px, double& py, double& dtheta, int max_crossings,
+                                       int lane, int* quarter_out) {
+  if constexpr (LANES == 1) {
+    (void)lane;
+    return seam_wrap(ax, ay, px, py, dtheta, max_crossings, nullptr, quarter_out);
+  } else {
+    const int k = LANES >= 4 ? (lane & 3) : (lane & 1);
+    const int base = (int)(__lane_id() & ~(unsigned)(LANES - 1));
+    int crossings = 0;
+    int quarter = 0;
+    dtheta = 0.0;
+    bool ok = true;
+    while (!inside_unit(px, py)) {
+      if (crossings >= max_crossings) { ok = false; break; }
+      ++crossings;
+      const double dx = px - ax, dy = py - ay, omx = 1.0 - ax, omy = 1.0 - ay;
+      const double rx = quotient_reciprocal(dx), ry = quotient_reciprocal(-dy);
+      unsigned bits;
+      if constexpr (LANES >= 4) {
+        bits = (unsigned)(__ballot(border_test(k, ax, ay, dx, dy, omx, omy)) >> base) & 0xFu;
+      } else {
+        const unsigned lo = (unsigned)(__ballot(border_test(k, ax, ay, dx, dy, omx, omy)) >> base) & 0x3u;
+        const unsigned hi = (unsigned)(__ballot(border_test(k + 2, ax, ay, dx, dy, omx, omy)) >> base) & 0x3u;
+        bits = lo | (hi << 2);
+      }
+      const int border = bits ? __builtin_ctz(bits) : 4;
+      cross_border(border, ax, ay, px, py, dx, dy, omx, omy, rx, ry, dtheta, quarter);
+    }
+    if (quarter_out) *quarter_out = quarter & 3;
+    return ok;
+  }
 }
 
 // heading kept in [-pi, pi)

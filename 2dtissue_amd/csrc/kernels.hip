@@ -263,6 +263,25 @@ __global__ void k_prepare(int n, const float* x, const float* y, const float* th
   th_out[i] = th[i];
 }
 
+// seam wrap of n moves, LANES lanes per move (the step kernel's lane groups); whole groups only
+template <int LANES>
+__global__ void __launch_bounds__(256) k_seam_wrap(int n, const double2* start, double2* end, double* dtheta,
+                                                   int* quarter, int* ok, int max_crossings) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = t / LANES, lane = t & (LANES - 1);
+  if (i >= n) return;  // uniform over the group
+  const double2 a = start[i];
+  double px = end[i].x, py = end[i].y, dth = 0.0;
+  int q = 0;
+  const bool good = seam_wrap_group<LANES>(a.x, a.y, px, py, dth, max_crossings, lane, &q);
+  if (lane == 0) {
+    end[i] = make_double2(px, py);
+    dtheta[i] = dth;
+    quarter[i] = q;
+    ok[i] = good ? 1 : 0;
+  }
+}
+
 __global__ void k_locate(MeshDev mesh, int n, const float4* xycs, int* vid, int* face, float* r3x, float* r3y,
                          float* r3z) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -365,6 +384,20 @@ hipError_t launch_prepare(int n, const float* x, const float* y, const float* th
 hipError_t launch_locate(const MeshDev& mesh, int n, const float4* xycs, int* vid, int* face, float* r3x, float* r3y,
                          float* r3z, hipStream_t st) {
   hipLaunchKernelGGL(k_locate, dim3(cdiv(n, 256)), dim3(256), 0, st, mesh, n, xycs, vid, face, r3x, r3y, r3z);
+  return hipGetLastError();
+}
+
+hipError_t launch_seam_wrap(int lanes, int n, const double2* start, double2* end, double* dtheta, int* quarter, int* ok,
+                            int max_crossings, hipStream_t st) {
+  const dim3 grid(cdiv(n * lanes, 256)), block(256);
+  switch (lanes) {
+    case 1: hipLaunchKernelGGL(k_seam_wrap<1>, grid, block, 0, st, n, start, end, dtheta, quarter, ok, max_crossings); break;
+    case 2: hipLaunchKernelGGL(k_seam_wrap<2>, grid, block, 0, st, n, start, end, dtheta, quarter, ok, max_crossings); break;
+    case 4: hipLaunchKernelGGL(k_seam_wrap<4>, grid, block, 0, st, n, start, end, dtheta, quarter, ok, max_crossings); break;
+    case 8: hipLaunchKernelGGL(k_seam_wrap<8>, grid, block, 0, st, n, start, end, dtheta, quarter, ok, max_crossings); break;
+    case 16: hipLaunchKernelGGL(k_seam_wrap<16>, grid, block, 0, st, n, start, end, dtheta, quarter, ok, max_crossings); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -200,6 +200,18 @@ int32_t tissue_adjoint(tissue_sim* s, int32_t n_steps, int32_t first_row, int32_
 int32_t tissue_batch_adjoint(tissue_batch* b, int32_t n_steps, int32_t first_row, int32_t ckpt_every,
                              const tissue_loss* loss, tissue_grads* g, double* d_theta0, double* d_x0, double* d_y0);
 
+/* ---- seam wrap on its own ---- */
+/* Applies the seam wrap of the step to n moves on the GPU, as
+ * EuclideanTiling::diagonal_seam_edges_square_border (EuclideanTiling.cpp:31-69) does to r_UV_old
+ * -> r_UV: start_xy[2n] are the positions before the move (in the unit square), end_xy[2n] the
+ * positions after it, overwritten with the wrapped positions; dtheta[n] receives the heading
+ * rotation (radians), quarter[n] (may be NULL) the quarter turns of the composed map. lanes (1, 2,
+ * 4, 8, 16) is the number of threads per move, as in the step kernel; max_crossings <= 0 selects
+ * 16384. Returns TISSUE_E_TILING_NO_CONVERGE when a move needs more crossings (its outputs are
+ * those reached at the bound). Host arrays; runs synchronously on the current device. */
+int32_t tissue_seam_wrap(int32_t n, const double* start_xy, double* end_xy, double* dtheta, int32_t* quarter,
+                         int32_t max_crossings, int32_t lanes);
+
 /* ---- host helpers (no GPU) ---- */
 /* eta[t][i] = std * N(0,1) from Philox4x32-10 (Salmon et al., SC'11) + Box-Muller */
 int32_t tissue_gen_noise(uint64_t seed, double std, int32_t n_steps, int32_t n, float* out);
