@@ -89,8 +89,9 @@ struct StepOut {
   int q;                     // seam wrap: counter-clockwise quarter turns of the composed map
 };
 
-// table entries per lane loaded together in the neighbour walk
+// table entries per lane loaded together in the neighbour walk (scan phase, and the FP64 redo)
 constexpr int kRowChunk = 4;
+constexpr int kScanChunk = 8;
 
 // Global-memory neighbour source: the previous step's SoA state and vertex buckets.
 struct GlobalSource {
@@ -107,6 +108,7 @@ struct GlobalSource {
     h.s1 = b.slot[(size_t)w * kSlots + 1];
     return h;
   }
+  __device__ __forceinline__ int count(int w) const { return b.count[w]; }
   template <class F>
   __device__ __forceinline__ void each(int w, const Head& h, F&& f) const {
     if (h.c == 0) return;
@@ -133,6 +135,7 @@ struct LdsSource {
     h.c = (int)begin[w + 1] - h.b;
     return h;
   }
+  __device__ __forceinline__ int count(int w) const { return (int)begin[w + 1] - (int)begin[w]; }
   template <class F>
   __device__ __forceinline__ void each(int, const Head& h, F&& f) const {
     const int b = h.b, c = h.c;
@@ -175,34 +178,49 @@ __device__ __forceinline__ StepOut particle_step(const MeshDev& mesh, const Step
   const double kc64 = prm.k64 / prm.two_sigma64;  // Fij / d = kc (d - 2 sigma) / d, to a few ulps
   int na = 0, nb = 0;
   const int rs = mesh.row_ptr[v], re = mesh.row_ptr[v + 1];
-  // Each lane walks its entries rs + lane + m * LANES in order, kRowChunk at a time: the chunk's
-  // table entries and bucket heads are loaded together, so the walk waits on one load latency per
-  // chunk rather than one per entry (same order, same sums).
-  for (int base = rs + lane; base < re; base += kRowChunk * LANES) {
-    int2 ent[kRowChunk];
-    typename Src::Head head[kRowChunk];
+  // Each lane walks its entries rs + lane + m * LANES, m = 0, 1, .., in order, in windows of 32:
+  //  (1) scan: the window's table entries and their bucket counts are loaded kScanChunk at a time
+  //      (one load latency per chunk), counted-only entries (2 sigma <= d <= 2.4 sigma) add their
+  //      count to nb, and near entries whose bucket holds particles are marked;
+  //  (2) visit: only the marked entries, in order, set up their pair coefficient and add their
+  //      particles. Most table entries have empty buckets, so the lanes of a wave loop over a few
+  //      occupied entries instead of paying the pair set-up for every entry; the order, and so
+  //      every sum, is that of a walk over all entries.
+  bool done = false;
+  for (int wbase = rs + lane; !done && wbase < re; wbase += 32 * LANES) {
+    unsigned mark = 0u;
+    for (int m0 = 0; m0 < 32 && !done; m0 += kScanChunk) {
+      int2 ent[kScanChunk];
+      int cnt[kScanChunk];
 #pragma unroll
-    for (int k = 0; k < kRowChunk; ++k) {
-      const int e = base + k * LANES;
-      ent[k] = e < re ? mesh.nbr[e] : make_int2(0, 0x7f800000);  // +inf ends the walk
-    }
-#pragma unroll
-    for (int k = 0; k < kRowChunk; ++k)
-      if (__int_as_float(ent[k].y) <= prm.t_count) head[k] = src.head(ent[k].x);
-    bool stop = false;
-#pragma unroll
-    for (int k = 0; k < kRowChunk; ++k) {
-      const float d = __int_as_float(ent[k].y);
-      if (d > prm.t_count) { stop = true; break; }  // rows ascend in distance
-      if (d > prm.t_near) {                         // counted only (2 sigma <= d <= 2.4 sigma)
-        nb += head[k].c;
-        continue;
+      for (int k = 0; k < kScanChunk; ++k) {
+        const int e = wbase + (m0 + k) * LANES;
+        ent[k] = e < re ? mesh.nbr[e] : make_int2(0, 0x7f800000);  // +inf ends the walk
       }
+#pragma unroll
+      for (int k = 0; k < kScanChunk; ++k) cnt[k] = __int_as_float(ent[k].y) <= prm.t_count ? src.count(ent[k].x) : 0;
+#pragma unroll
+      for (int k = 0; k < kScanChunk; ++k) {
+        const float d = __int_as_float(ent[k].y);
+        if (d > prm.t_count) {  // rows ascend in distance
+          done = true;
+          break;
+        }
+        if (d > prm.t_near) nb += cnt[k];  // counted only
+        else if (cnt[k] > 0) mark |= 1u << (m0 + k);
+      }
+    }
+    while (mark) {
+      const int m = __builtin_ctz(mark);
+      mark &= mark - 1u;
+      const int2 en = mesh.nbr[wbase + m * LANES];
+      const typename Src::Head head = src.head(en.x);
+      const float d = __int_as_float(en.y);
       const float g = pair_coef(d, prm);
       const double dd = d == 0.0f ? 0.001 : (double)d;
       const double g64 = (kc64 * (dd - prm.two_sigma64)) / dd;
       const bool counted = d != 0.0f;
-      src.each(ent[k].x, head[k], [&](int j) {
+      src.each(en.x, head, [&](int j) {
         const float4 q = src.pos(j);
         if (j != i) {
           Fx += g * (me.x - q.x);
@@ -216,7 +234,6 @@ __device__ __forceinline__ StepOut particle_step(const MeshDev& mesh, const Step
         if (counted) ++nb;
       });
     }
-    if (stop) break;
   }
   Fx = group_sum<LANES>(Fx);
   Fy = group_sum<LANES>(Fy);
