@@ -50,13 +50,44 @@ def log(msg: str) -> None:
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
-def roofline(particles: int, avg_launch_s: float, kernel: str, steps_per_launch: int = 1) -> dict:
+COUNTERS = os.path.join(ROOT, "profiles", "step_kernel_counters.json")
+
+
+def measured(workload: str | None, launch: dict | None):
+    """rocprofv3 counters recorded for this workload's step kernel (profiles/step_kernel_counters.json:
+    FETCH_SIZE / WRITE_SIZE scaled by the copy calibration, VALU instruction issue), used only when
+    the launch shape is the profiled one."""
+    if workload is None or launch is None:
+        return None
+    try:
+        with open(COUNTERS) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    m = d.get("workloads", {}).get(workload)
+    if not m or launch.get("lanes") != m["lanes"] or launch.get("block") != m["block"]:
+        return None
+    return m, d.get("source_commit")
+
+
+def roofline(particles: int, avg_launch_s: float, kernel: str, steps_per_launch: int = 1,
+             workload: str | None = None, launch: dict | None = None) -> dict:
+    """HBM roofline of one step: algorithmic bytes (64 B per particle-step: the state read and
+    written) over the step's average duration from HIP events on the stream the kernels run on."""
     bytes_per_launch = BYTES_PER_PARTICLE_STEP * particles * steps_per_launch
     achieved = bytes_per_launch / avg_launch_s / 1e9
-    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel,
-            "avg_launch_us": avg_launch_s * 1e6, "bytes_per_launch": bytes_per_launch,
-            "steps_per_launch": steps_per_launch}
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel,
+           "avg_launch_us": avg_launch_s * 1e6, "bytes_per_launch": bytes_per_launch,
+           "steps_per_launch": steps_per_launch}
+    m = measured(workload, launch)
+    if m:
+        c, commit = m
+        out["traffic"] = c["traffic_bytes"]
+        out["traffic_kernel"] = c["kernel"]
+        out["valu_issue_frac"] = c["valu_issue_frac"]
+        out["counters_source"] = f"profiles/step_kernel_counters.json (commit {commit})"
+    return out
 
 
 def cpu_baseline(tables, x, y, th, vid, noise, steps, threads=1):
@@ -96,7 +127,7 @@ def plain(T, mesh, mode, lanes, steps, warmup):
     sim.synchronize()
     elapsed = time.perf_counter() - t0
     res = dict(code=code, elapsed=elapsed, kernel_ms=sim.last_step_ms(), flags=sim.flags()[0],
-               phi_polar_last=float(sim.order()[1][-1]))
+               phi_polar_last=float(sim.order()[1][-1]), launch=sim.launch())
     sim.close()
     return res, s0, noise_h
 
@@ -133,8 +164,9 @@ def batched(T, mesh, steps, n_replicas=256, plain_sample=16):
                elapsed_s=elapsed, graph_ms=run_ms, status=code, replicas_flagged=int((flags != 0).sum()),
                launch=shape,
                roofline=roofline(b.ntot, run_ms / 1e3 / steps,
-                                 "k_step_tiled (one step, all replicas)" if shape["mode"] == 1
-                                 else "k_resident (one step, all replicas)"))
+                                 "k_step_tiled + k_bucket_finalize (one step, all replicas)" if shape["mode"] == 1
+                                 else "k_resident (one step, all replicas)",
+                                 workload="batched" if shape["mode"] == 1 else None, launch=shape))
     b.close()
     # the same replicas, each launched alone through the plain per-step kernel
     pick = np.argsort(sizes)[np.linspace(0, n_replicas - 1, plain_sample).astype(int)]
@@ -211,8 +243,10 @@ def large(T, steps=500, adj_steps=200):
     code = sim.step_staged(steps, 20)
     sim.synchronize()
     el = time.perf_counter() - t0
-    fwd = dict(value=n * steps / el, unit="particle-steps/s", status=code, elapsed_s=el, launch=sim.launch(),
-               roofline=roofline(n, sim.last_step_ms() / 1e3 / steps, "k_step_tiled"))
+    shape = sim.launch()
+    fwd = dict(value=n * steps / el, unit="particle-steps/s", status=code, elapsed_s=el, launch=shape,
+               roofline=roofline(n, sim.last_step_ms() / 1e3 / steps, "k_step_tiled + k_bucket_finalize (one step)",
+                                 workload="large", launch=shape))
     sim.close()
     R, nr = 64, 2000
     rng = np.random.default_rng(6)
@@ -304,7 +338,9 @@ def main():
             "particles": n, "faces": tables["nf"], "uv_vertices": tables["nv"], "status": main_res["code"],
             "flags": main_res["flags"],
         },
-        "roofline": roofline(n, main_res["kernel_ms"] / 1e3 / args.steps, "k_step_tiled"),
+        "roofline": roofline(n, main_res["kernel_ms"] / 1e3 / args.steps, "k_step_tiled + k_bucket_finalize (one step)",
+                             workload="plain", launch=main_res["launch"]),
+        "launch": main_res["launch"],
         "plain_resident_kernel": {
             "value": n * args.steps / res_res["elapsed"], "unit": "particle-steps/s",
             "ms_per_step": res_res["elapsed"] * 1e3 / args.steps, "status": res_res["code"],
