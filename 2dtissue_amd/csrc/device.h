@@ -166,15 +166,16 @@ struct LdsSource {
 // Neighbours: particles bucketed on the vertices w of row v of the geodesic table; pair
 // distance d = D[v, w]. Force and alignment for d < 2 sigma (ForceHelper.cpp:34-73,
 // OrientationHelper.cpp:29-74, self included in the mean), count for 0 < d <= 2.4 sigma
-// (2DTissue.cpp:254-268). FP32 sums, with the force also summed in FP64 (used when the
-// displacement is large); the alignment sum is redone in FP64 when it nearly cancels.
+// (2DTissue.cpp:254-268). The force is summed in FP64 (its FP32 rounding feeds the FP32 speed
+// and Euler step; the FP64 value the large-displacement path); the alignment sums are FP32, redone
+// in FP64 when they nearly cancel.
 // Always inlined: as an out-of-line call (which the inliner picks once two kernels share an
 // instantiation) its result and live state go through scratch memory.
 template <int LANES, class Src>
 __device__ __forceinline__ StepOut particle_step(const MeshDev& mesh, const StepParams& prm, const Src& src, int i, float4 me,
                                         float th, int v, float eta, int lane) {
-  float Fx = 0.f, Fy = 0.f, Sx = 0.f, Sy = 0.f;
-  double Fx64 = 0.0, Fy64 = 0.0;  // the same force in FP64 (used when the displacement is large)
+  float Sx = 0.f, Sy = 0.f;
+  double Fx64 = 0.0, Fy64 = 0.0;  // the force, summed in FP64 (its FP32 rounding drives the FP32 path)
   const double kc64 = prm.k64 / prm.two_sigma64;  // Fij / d = kc (d - 2 sigma) / d, to a few ulps
   int na = 0, nb = 0;
   const int rs = mesh.row_ptr[v], re = mesh.row_ptr[v + 1];
@@ -216,15 +217,12 @@ __device__ __forceinline__ StepOut particle_step(const MeshDev& mesh, const Step
       const int2 en = mesh.nbr[wbase + m * LANES];
       const typename Src::Head head = src.head(en.x);
       const float d = __int_as_float(en.y);
-      const float g = pair_coef(d, prm);
       const double dd = d == 0.0f ? 0.001 : (double)d;
       const double g64 = (kc64 * (dd - prm.two_sigma64)) / dd;
       const bool counted = d != 0.0f;
       src.each(en.x, head, [&](int j) {
         const float4 q = src.pos(j);
         if (j != i) {
-          Fx += g * (me.x - q.x);
-          Fy += g * (me.y - q.y);
           Fx64 += g64 * ((double)me.x - (double)q.x);
           Fy64 += g64 * ((double)me.y - (double)q.y);
         }
@@ -235,8 +233,9 @@ __device__ __forceinline__ StepOut particle_step(const MeshDev& mesh, const Step
       });
     }
   }
-  Fx = group_sum<LANES>(Fx);
-  Fy = group_sum<LANES>(Fy);
+  Fx64 = group_sum<LANES>(Fx64);
+  Fy64 = group_sum<LANES>(Fy64);
+  float Fx = (float)Fx64, Fy = (float)Fy64;
   Sx = group_sum<LANES>(Sx);
   Sy = group_sum<LANES>(Sy);
   na = group_sum<LANES>(na);
@@ -254,9 +253,7 @@ __device__ __forceinline__ StepOut particle_step(const MeshDev& mesh, const Step
   const bool weak = !(Sm >= 0.6f * prm.rate * (float)na);
   const bool big = !(spd * prm.dt <= kBigStep);
   double thn;
-  if (big) {  // uniform over the particle's lanes: the FP32 sums were reduced over the group
-    Fx64 = group_sum<LANES>(Fx64);
-    Fy64 = group_sum<LANES>(Fy64);
+  if (big) {  // uniform over the particle's lanes: the sums were reduced over the group
     double st, ct;
     sincos((double)th, &st, &ct);
     const double spd64 = prm.v064 + prm.mu64 * sqrt(Fx64 * Fx64 + Fy64 * Fy64);
@@ -265,8 +262,6 @@ __device__ __forceinline__ StepOut particle_step(const MeshDev& mesh, const Step
     yn = (double)me.y + vy64 * prm.dt64;
     vx = (float)vx64;
     vy = (float)vy64;
-    Fx = (float)Fx64;
-    Fy = (float)Fy64;
   }
   if (weak) {
     // the alignment sum again, over the neighbours' headings in FP64
