@@ -90,6 +90,17 @@ def roofline(particles: int, avg_launch_s: float, kernel: str, steps_per_launch:
     return out
 
 
+def warm_graph(run, steps: int, row: int) -> None:
+    """Captures and instantiates, outside the timed region, the step graph that a timed
+    `run(steps, row)` will launch. The engine keys its graphs by step count, noise rows and the
+    phase of its rotating buffers (state: 2, vertex buckets: 3), so after one untimed call it runs
+    (-steps) % 6 more steps to bring the phase back."""
+    run(steps, row)
+    extra = (-steps) % 6
+    if extra:
+        run(extra, row)
+
+
 def cpu_baseline(tables, x, y, th, vid, noise, steps, threads=1):
     """The oracle's FP64 dense reference algorithm (N x N matrices, brute-force remap over all
     faces, restart tiling loop), timed on this host for a bounded number of steps."""
@@ -121,6 +132,7 @@ def plain(T, mesh, mode, lanes, steps, warmup):
     sim.stage_noise(noise_h)
     if warmup:
         sim.step_staged(warmup, 0)
+    warm_graph(sim.step_staged, steps, warmup)
     sim.synchronize()
     t0 = time.perf_counter()
     code = sim.step_staged(steps, warmup)
@@ -151,7 +163,7 @@ def batched(T, mesh, steps, n_replicas=256, plain_sample=16):
     b.autotune(20)
     shape = b.launch()
     log(f"batched: {shape}; timing {steps} steps")
-    b.run(steps, 0)  # warm the graph (captured and instantiated on first use)
+    warm_graph(b.run, steps, 0)
     b.init_random(init_seeds)
     t0 = time.perf_counter()
     code = b.run(steps, 0)
@@ -177,6 +189,8 @@ def batched(T, mesh, steps, n_replicas=256, plain_sample=16):
         s.set_mode(T.MODE_STEP_KERNEL)
         s.init_random(int(init_seeds[r]))
         s.stage_noise(T.gen_noise(int(noise_seeds[r]), float(std[r]), steps, int(sizes[r])))
+        warm_graph(s.step_staged, steps, 0)
+        s.init_random(int(init_seeds[r]))
         s.synchronize()
         t0 = time.perf_counter()
         s.step_staged(steps, 0)
@@ -237,6 +251,7 @@ def large(T, steps=500, adj_steps=200):
     sim.init_random(5)
     sim.stage_noise(T.gen_noise(5, 0.1, steps + 20, n))
     sim.step_staged(20, 0)
+    warm_graph(sim.step_staged, steps, 20)
     sim.synchronize()
     log(f"large surface: {n} particles, {steps} steps")
     t0 = time.perf_counter()
@@ -291,7 +306,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--lanes", type=int, default=0)
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--batched-steps", type=int, default=500)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
