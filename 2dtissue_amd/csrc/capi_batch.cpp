@@ -77,8 +77,8 @@ int32_t tissue_batch_gen_noise(tissue_batch* b, const uint64_t* seeds, const dou
 int32_t tissue_batch_run(tissue_batch* b, int32_t n_steps, int32_t first_row) {
   return guarded([&] {
     auto& e = eng(b);
-    const float* nz = first_row >= 0 ? e.staged_rows(first_row, n_steps) : nullptr;
-    return e.run(n_steps, nz, (size_t)e.ntot);
+    if (first_row < 0) return e.run(n_steps, nullptr, 0);
+    return e.run(n_steps, e.staged_rows(0, 0), (size_t)e.ntot, e.checked_first_row(first_row, n_steps));
   });
 }
 

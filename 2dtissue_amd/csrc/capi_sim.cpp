@@ -91,7 +91,7 @@ int32_t tissue_stage_noise(tissue_sim* s, const float* noise, int32_t n_rows) {
 int32_t tissue_step_staged(tissue_sim* s, int32_t n_steps, int32_t first_row) {
   return guarded([&] {
     auto& e = eng(s);
-    return e.run(n_steps, e.staged_rows(first_row, n_steps), (size_t)e.ntot);
+    return e.run(n_steps, e.staged_rows(0, 0), (size_t)e.ntot, e.checked_first_row(first_row, n_steps));
   });
 }
 

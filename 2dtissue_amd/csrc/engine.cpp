@@ -65,6 +65,9 @@ Engine::Engine(tissue_mesh* m, int R_, const int* n_, const tissue_params* p_) {
     HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIP_OK(hipEventCreate(&ev0));
     HIP_OK(hipEventCreate(&ev1));
+    d_row0 = dalloc<int>(1);
+    HIP_OK(hipHostMalloc((void**)&h_row0, sizeof(int), hipHostMallocDefault));
+    *h_row0 = 0;
     if (nmax <= resident_cap) HIP_OK(resident_prepare(md.nv, nmax));
     d_reps = dalloc<ReplicaDesc>(R);
     upload(d_reps, reps.data(), reps.size());
@@ -114,7 +117,9 @@ void Engine::release() {
     dfree(bs[k].count); dfree(bs[k].slot); dfree(bs[k].ovf_n); dfree(bs[k].ovf);
     dfree(bs[k].multi_n); dfree(bs[k].multi); dfree(bs[k].big_n); dfree(bs[k].big);
   }
-  dfree(d_tiles); dfree(d_tile_begin); dfree(partial); dfree(phi); dfree(noise); dfree(call_noise);
+  dfree(d_tiles); dfree(d_tile_begin); dfree(partial); dfree(phi); dfree(noise); dfree(call_noise); dfree(d_row0);
+  if (h_row0) (void)hipHostFree(h_row0);
+  h_row0 = nullptr;
   adjoint_release();
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
@@ -328,6 +333,11 @@ const float* Engine::staged_rows(int first_row, int n_steps) const {
   return noise + (size_t)first_row * ntot;
 }
 
+int Engine::checked_first_row(int first_row, int n_steps) const {
+  (void)staged_rows(first_row, n_steps);
+  return first_row;
+}
+
 const float* Engine::upload_call_noise(const float* h, int n_steps) {
   const size_t need = (size_t)std::max(n_steps, 0) * ntot;
   if (call_noise_cap < need) {
@@ -407,6 +417,7 @@ void Engine::enqueue(const Shape& s, int n_steps, const float* nz, size_t stride
     for (int t0 = 0; t0 < n_steps; t0 += s.spl) {
       const int k = std::min(s.spl, n_steps - t0);
       io.noise = nz ? nz + (size_t)t0 * stride : nullptr;
+      io.noise_row0 = d_row0;
       io.t_base = t0;
       io.flags_in = d_rows + (size_t)(t0 % 3) * R;
       io.flags_out = d_rows + (size_t)((t0 + k) % 3) * R;
@@ -424,6 +435,8 @@ void Engine::enqueue(const Shape& s, int n_steps, const float* nz, size_t stride
         io.vx = vx; io.vy = vy; io.r3x = r3x; io.r3y = r3y; io.r3z = r3z;
         io.face = face; io.nbr = nbr;
         io.noise = nz ? nz + (size_t)t * stride : nullptr;
+        io.noise_row0 = d_row0;
+        io.noise_stride = stride;
         io.partial = partial + (size_t)q * n_logical;
         io.halt_in = d_rows + (size_t)(t % 3) * R;
         io.halt_out = d_rows + (size_t)((t + 1) % 3) * R;
@@ -442,9 +455,11 @@ void Engine::enqueue(const Shape& s, int n_steps, const float* nz, size_t stride
   HIP_OK(launch_flags_end(R, d_rows + (size_t)(n_steps % 3) * R, d_halt, stream));
 }
 
-int32_t Engine::run(int n_steps, const float* nz, size_t stride) { return run_shape(resolve(), n_steps, nz, stride); }
+int32_t Engine::run(int n_steps, const float* nz, size_t stride, int row0) {
+  return run_shape(resolve(), n_steps, nz, stride, row0);
+}
 
-int32_t Engine::run_shape(const Shape& s, int n_steps, const float* nz, size_t stride) {
+int32_t Engine::run_shape(const Shape& s, int n_steps, const float* nz, size_t stride, int row0) {
   if (!has_state) throw ApiError(TISSUE_E_INVALID_ARG, "no state: call init_random or set_state first");
   if (n_steps < 0) throw ApiError(TISSUE_E_INVALID_ARG, "negative step count");
   if (n_steps == 0) return TISSUE_OK;
@@ -492,6 +507,8 @@ int32_t Engine::run_shape(const Shape& s, int n_steps, const float* nz, size_t s
     HIP_OK(e);
     it = graphs.emplace(key, ge).first;
   }
+  *h_row0 = nz ? row0 : 0;  // the previous run has completed (synchronised below), so the pinned word is free
+  HIP_OK(hipMemcpyAsync(d_row0, h_row0, sizeof(int), hipMemcpyHostToDevice, stream));
   HIP_OK(hipEventRecord(ev0, stream));
   HIP_OK(hipGraphLaunch(it->second, stream));
   HIP_OK(hipEventRecord(ev1, stream));

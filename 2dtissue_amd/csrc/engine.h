@@ -44,10 +44,14 @@ struct Engine {
   const float* staged_rows(int first_row, int n_steps) const;
   // one call's rows in a separate buffer (leaves the staged rows alone)
   const float* upload_call_noise(const float* h, int n_steps);
+  // first_row, checked against the staged rows (throws when rows first_row .. + n_steps are absent)
+  int checked_first_row(int first_row, int n_steps) const;
 
   // n_steps steps; row s of the noise at noise + s * stride (null: no noise). Returns the status.
-  int32_t run(int n_steps, const float* noise, size_t stride);
-  int32_t run_shape(const Shape& s, int n_steps, const float* noise, size_t stride);
+  // row0: first row of the noise used (row s of the run at noise + (row0 + s) * stride); the
+  // captured step graphs do not depend on it (it is read from device memory)
+  int32_t run(int n_steps, const float* noise, size_t stride, int row0 = 0);
+  int32_t run_shape(const Shape& s, int n_steps, const float* noise, size_t stride, int row0 = 0);
   Shape resolve() const;
   // the request resolved for the tiled step kernel (the adjoint records its tape there)
   Shape resolve_tiled() const;
@@ -71,6 +75,8 @@ struct Engine {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int* d_row0 = nullptr;  // first noise row of the current run, read by the step kernels
+  int* h_row0 = nullptr;  // pinned host copy it is uploaded from
   int R = 0, ntot = 0, nmax = 0, resident_cap = 0;
   std::vector<int> n, off;
   std::vector<tissue_params> p;

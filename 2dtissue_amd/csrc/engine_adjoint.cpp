@@ -116,7 +116,7 @@ int32_t Engine::adjoint(int T, const float* nz, size_t stride, int K, const tiss
     HIP_OK(hipMemcpyAsync(a.ck_xycs + c * N, xycs[sphase], N * sizeof(float4), hipMemcpyDeviceToDevice, stream));
     HIP_OK(hipMemcpyAsync(a.ck_th + c * N, th[sphase], N * sizeof(float), hipMemcpyDeviceToDevice, stream));
     HIP_OK(hipMemcpyAsync(a.ck_vid + c * N, vid[sphase], N * sizeof(int), hipMemcpyDeviceToDevice, stream));
-    const int32_t st = run_shape(s, kc, nz ? nz + (size_t)t0 * stride : nullptr, stride);
+    const int32_t st = run_shape(s, kc, nz, stride, t0);
     fwd_ms += last_ms;
     if (st != TISSUE_OK && status == TISSUE_OK) status = st;
   }

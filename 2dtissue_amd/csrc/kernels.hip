@@ -108,7 +108,8 @@ __global__ void __launch_bounds__(kStepMaxBlock) k_step_tiled(MeshDev mesh, cons
         bucket_insert(bo, v, g);
       }
     } else {
-      const float eta = io.noise ? io.noise[g] : 0.0f;
+      const size_t shift = io.noise_row0 ? (size_t)io.noise_row0[0] * io.noise_stride : 0;
+      const float eta = io.noise ? io.noise[shift + g] : 0.0f;
       const GlobalSource src{replica_in(bin, r, nv, rep.off), io.xycs_in, io.th_in};
       const StepOut o = particle_step<LANES>(mesh, rep.prm, src, g, me, th, v, eta, lane);
       if (lane == 0) {

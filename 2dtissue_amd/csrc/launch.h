@@ -56,7 +56,9 @@ struct TiledIO {
   float* r3z;
   int* face;
   int* nbr;
-  const float* noise;       // eta of this step, [ntot] (may be null: zero noise)
+  const float* noise;       // eta of this step, [ntot] (may be null: zero noise), shifted by
+  const int* noise_row0;    //   noise_row0[0] rows of noise_stride (null: no shift); lets one
+  size_t noise_stride;      //   captured graph serve any first noise row
   OrderPartial* partial;    // [logical tiles] of this step
   const unsigned* halt_in;  // [R] flags raised before this step (a flagged replica passes through)
   unsigned* halt_out;       // [R] flags raised up to and including this step
@@ -81,8 +83,9 @@ struct ResidentIO {
   float* r3z;
   int* face;
   int* nbr;
-  const float* noise;     // eta of the launch's first step; step s at noise + s * noise_stride
-  size_t noise_stride;
+  const float* noise;     // eta of the launch's first step; step s at noise + s * noise_stride,
+  size_t noise_stride;    // shifted by noise_row0[0] rows (null: no shift)
+  const int* noise_row0;
   double* phi;            // (phi_ref, phi_polar) of replica r, run step t at phi[2 * (r * phi_stride + t)]
   int phi_stride;
   int t_base;             // run step index of the launch's first step

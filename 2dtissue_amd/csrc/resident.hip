@@ -126,7 +126,8 @@ __global__ void __launch_bounds__(kMaxBlock) k_resident(MeshDev mesh, const Repl
       continue;
     }
     build_buckets(begin, sorted, vid, n, mesh.nv, scan_tmp);
-    const float* eta_row = io.noise ? io.noise + (size_t)s * io.noise_stride + off : nullptr;
+    const size_t row = (size_t)s + (io.noise_row0 ? (size_t)io.noise_row0[0] : 0);
+    const float* eta_row = io.noise ? io.noise + row * io.noise_stride + off : nullptr;
     int sgn = 0;
     double cs = 0.0, sn = 0.0;
     unsigned fl = 0;

@@ -44,3 +44,25 @@ def test_c1_teacher_forced_50_steps(T, O, lanes, mode):
 def test_c2_teacher_forced_20_steps(T, O, mode):
     total = run_teacher_forced(T, O, P.C2, 20, 0, mode)
     assert all(v == 0 for v in total.values()), total
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_staged_noise_rows_match_host_noise(T, mode):
+    """Staged noise read from a moving first row (the captured step graphs take the row from device
+    memory) gives, bit for bit, the trajectory of the same rows passed from the host."""
+    cfg = P.C2
+    m, _ = P.mesh_and_tables(cfg["freq"], cfg["sigma"])
+    p = T.params(**dict(P.PHYS, sigma=cfg["sigma"]))
+    noise = T.gen_noise(cfg["noise_seed"], 0.1, 40, cfg["n"])
+    a, b = T.Sim(m, p, cfg["n"]), T.Sim(m, p, cfg["n"])
+    for s in (a, b):
+        s.set_mode(mode)
+        s.init_random(cfg["seed"])
+    a.stage_noise(noise)
+    for row in (3, 11, 29, 11):  # the same step count from different rows, one row twice
+        assert a.step_staged(6, row) == T.OK
+        assert b.step(6, noise[row:row + 6]) == T.OK
+        sa, sb = a.get_state(), b.get_state()
+        for k in ("x", "y", "theta", "vid"):
+            assert np.array_equal(getattr(sa, k), getattr(sb, k)), (row, k)
+    a.close(); b.close()
