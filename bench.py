@@ -41,6 +41,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_PARTICLE_STEP = 64
+FP32_PEAK_GFLOPS = 157300.0  # MI355X FP32 vector (packed); 78,600 unpacked
+# algorithmic flops of one particle-step (transcendentals count 1): per neighbour pair within
+# 2 sigma, per particle (speed, Euler step, heading), per candidate face of the remap
+FLOP_PER_PAIR, FLOP_PER_PARTICLE, FLOP_PER_FACE = 20, 40, 40
 
 WORKLOAD = dict(freq=32, sigma=0.05, n=2000, seed=1, noise_seed=1, noise_std=0.1, v0=1.0, k=10.0, mu=1.0, tau=1.0,
                 dt=0.02)
@@ -90,6 +94,28 @@ def roofline(particles: int, avg_launch_s: float, kernel: str, steps_per_launch:
     return out
 
 
+def valu_axis(ps: float, pairs_per_particle: float, faces_per_locate: float, issue_frac) -> dict:
+    """Secondary axis: FP32 VALU throughput from the algorithmic flop count of a particle-step
+    (FLOP_PER_PAIR x neighbour pairs + FLOP_PER_PARTICLE + FLOP_PER_FACE x candidate faces)."""
+    flops = FLOP_PER_PAIR * pairs_per_particle + FLOP_PER_PARTICLE + FLOP_PER_FACE * faces_per_locate
+    achieved = ps * flops / 1e9
+    return {"bound": "fp32_valu", "achieved": achieved, "peak": FP32_PEAK_GFLOPS, "unit": "GFLOP/s",
+            "frac": achieved / FP32_PEAK_GFLOPS, "flop_per_particle_step": flops,
+            "pairs_per_particle": pairs_per_particle, "faces_per_locate": faces_per_locate,
+            "valu_issue_frac": issue_frac}
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def warm_graph(run, steps: int, row: int) -> None:
     """Captures and instantiates, outside the timed region, the step graph that a timed
     `run(steps, row)` will launch. The engine keys its graphs by step count, noise rows and the
@@ -116,6 +142,7 @@ def cpu_baseline(tables, x, y, th, vid, noise, steps, threads=1):
         x, y, th, vid = s["x"], s["y"], s["theta"], s["vid"]
     dt = time.perf_counter() - t0
     return dict(value=w["n"] * steps / dt, unit="particle-steps/s", cores=threads, kind="port",
+                cpu_model=cpu_model(),
                 sample=f"{steps} steps of the {w['n']}-particle workload (FP64, dense N^2 forces/alignment, "
                        f"brute-force remap over all {tables['nf']} faces), {dt:.1f} s wall")
 
@@ -138,8 +165,9 @@ def plain(T, mesh, mode, lanes, steps, warmup):
     code = sim.step_staged(steps, warmup)
     sim.synchronize()
     elapsed = time.perf_counter() - t0
+    pairs = float(np.mean([len(a) for a in sim.neighbor_lists()]))  # |A_i| of the final state
     res = dict(code=code, elapsed=elapsed, kernel_ms=sim.last_step_ms(), flags=sim.flags()[0],
-               phi_polar_last=float(sim.order()[1][-1]), launch=sim.launch())
+               phi_polar_last=float(sim.order()[1][-1]), launch=sim.launch(), pairs=pairs)
     sim.close()
     return res, s0, noise_h
 
@@ -234,7 +262,7 @@ def adjoint(T, mesh, n=1000, steps=200, ckpt=20):
                 max_abs_d_theta0=float(np.abs(dth).max()), roofline=r)
 
 
-def large(T, steps=500, adj_steps=200):
+def large(T, steps=500, adj_steps=200, cpu_threads=0):
     """The large surface: perturbed sphere r = 1 + 0.2 Y_3^2 (icosphere frequency 100, F = 200,000),
     sigma = 0.01. 100,000 particles (seed 5, noise seed 5) for `steps` forward steps; then 64
     replicas of 2,000 particles (tau log-uniform in [0.1, 10], seed 6) forward + adjoint together
@@ -253,6 +281,7 @@ def large(T, steps=500, adj_steps=200):
     sim.step_staged(20, 0)
     warm_graph(sim.step_staged, steps, 20)
     sim.synchronize()
+    s_cpu = sim.get_state() if cpu_threads else None
     log(f"large surface: {n} particles, {steps} steps")
     t0 = time.perf_counter()
     code = sim.step_staged(steps, 20)
@@ -263,6 +292,31 @@ def large(T, steps=500, adj_steps=200):
                roofline=roofline(n, sim.last_step_ms() / 1e3 / steps, "k_step_tiled + k_bucket_finalize (one step)",
                                  workload="large", launch=shape))
     sim.close()
+    cpu = None
+    if cpu_threads:
+        log(f"large surface: CPU, one step of the sparse oracle on {cpu_threads} threads")
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        O.set_threads(cpu_threads)
+        om = O.Mesh(mesh.export())
+        eta = T.gen_noise(5, 0.1, 200, n).astype(np.float64)
+        po = dict(sigma=0.01, v0=w["v0"], k=w["k"], mu=w["mu"], tau=w["tau"], dt=w["dt"])
+        x, y, th, vid = (s_cpu.x.astype(np.float64), s_cpu.y.astype(np.float64), s_cpu.theta.astype(np.float64),
+                         s_cpu.vid)
+        t0 = time.perf_counter()
+        done, status = 0, 0
+        while done < len(eta) and time.perf_counter() - t0 < 10.0:  # bounded sample: about 10 s
+            status, st_, _ = O.step_sparse(om, po, x, y, th, vid, eta[done])
+            x, y, th, vid = st_["x"], st_["y"], st_["theta"], st_["vid"]
+            done += 1
+            if status != O.OK:
+                break
+        dt = time.perf_counter() - t0
+        cpu = dict(value=n * done / dt, unit="particle-steps/s", cores=cpu_threads, kind="port", cpu_model=cpu_model(),
+                   status=int(status), sample=f"{done} steps of the {n}-particle state, {dt:.1f} s wall",
+                   algorithm="scalable CPU algorithm: the oracle's sparse O(N) restatement (geodesic-table rows "
+                             "instead of N x N matrices, bitwise equal to the dense one); the reference's dense "
+                             "algorithm is infeasible here (N x N matrices of 80 GB)")
     R, nr = 64, 2000
     rng = np.random.default_rng(6)
     taus = np.exp(rng.uniform(np.log(0.1), np.log(10.0), R))
@@ -283,6 +337,7 @@ def large(T, steps=500, adj_steps=200):
                forward_ms=gs[0]["forward_ms"], reverse_ms=gs[0]["reverse_ms"],
                d_tau_first=[g["d_tau"] for g in gs[:4]])
     return dict(mesh=dict(faces=info["nf"], uv_vertices=info["nv"], table_entries=info["nnz"], build_s=build_s),
+                cpu_baseline_scalable=cpu,
                 forward=dict(workload=f"{n} particles, sigma 0.01, {steps} steps", **fwd),
                 batched_adjoint=dict(workload=f"{R} x {nr} particles, {adj_steps} steps, checkpoints every 20", **adj))
 
@@ -326,6 +381,7 @@ def main():
     log("building mesh tables")
     mesh = T.Mesh.ellipsoid(2.0, 1.0, 1.0, w["freq"]).build_tables(w["sigma"])
     tables = mesh.export()
+    info = mesh.info()
 
     log(f"plain, default mode: warmup {args.warmup}, timing {args.steps} steps")
     main_res, s0, noise_h = plain(T, mesh, T.MODE_AUTO, args.lanes, args.steps, args.warmup)
@@ -355,6 +411,8 @@ def main():
         },
         "roofline": roofline(n, main_res["kernel_ms"] / 1e3 / args.steps, "k_step_tiled + k_bucket_finalize (one step)",
                              workload="plain", launch=main_res["launch"]),
+        "valu": valu_axis(ps, main_res["pairs"], info["n_cell_faces"] / info["grid"] ** 2,
+                          (measured("plain", main_res["launch"]) or ({},))[0].get("valu_issue_frac")),
         "launch": main_res["launch"],
         "plain_resident_kernel": {
             "value": n * args.steps / res_res["elapsed"], "unit": "particle-steps/s",
@@ -367,12 +425,17 @@ def main():
     if not args.no_adjoint:
         log("forward + adjoint: 1000 particles, 200 steps")
         out["adjoint"] = adjoint(T, mesh)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
     if not args.no_large:
-        out["large"] = large(T)
+        out["large"] = large(T, cpu_threads=0 if args.no_cpu else threads)
     if not args.no_cpu:
         log(f"CPU baseline: {args.cpu_steps} oracle steps")
         out["cpu_baseline"] = cpu_baseline(tables, s0.x.astype(np.float64), s0.y.astype(np.float64),
                                            s0.theta.astype(np.float64), s0.vid, noise_h, args.cpu_steps)
+        log(f"CPU baseline on {threads} threads: {args.cpu_steps} oracle steps")
+        out["cpu_baseline_all_cores"] = cpu_baseline(tables, s0.x.astype(np.float64), s0.y.astype(np.float64),
+                                                     s0.theta.astype(np.float64), s0.vid, noise_h, args.cpu_steps,
+                                                     threads=threads)
     if calib:
         out["copy_calibration"] = calib
     print(json.dumps(out), flush=True)
