@@ -161,6 +161,7 @@ SIGNATURES = [
     ("tissue_gen_noise", C.c_int32, [C.c_uint64, C.c_double, C.c_int32, C.c_int32, _F]),
     ("tissue_philox4x32", None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     ("tissue_seam_wrap", C.c_int32, [C.c_int32, _D, _D, _D, _I, C.c_int32, C.c_int32]),
+    ("tissue_random_init_host", C.c_int32, [_P, C.c_uint64, C.c_int32, _F, _F, _F]),
 ]
 
 
@@ -226,6 +227,12 @@ class Mesh:
                                       C.byref(ncf)))
         return dict(nv=nv.value, nf=nf.value, nv3d=nv3d.value, nnz=nnz.value, grid=grid.value,
                     n_cell_faces=ncf.value)
+
+    def random_init_host(self, seed: int, n: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """The initial (x, y, theta) that Sim.init_random(seed) draws on the GPU, computed on the host."""
+        x = np.zeros(n, np.float32); y = np.zeros(n, np.float32); t = np.zeros(n, np.float32)
+        _check(lib().tissue_random_init_host(self.h, seed, n, _ptr(x, C.c_float), _ptr(y, C.c_float), _ptr(t, C.c_float)))
+        return x, y, t
 
     def export(self) -> dict:
         i = self.info()

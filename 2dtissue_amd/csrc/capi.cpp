@@ -18,7 +18,7 @@ using namespace tcapi;
 
 thread_local std::string tcapi::g_err;
 
-void tcapi::random_init(const tissue::UVMesh& M, uint64_t seed, int n, float* x, float* y, float* th) {
+std::vector<double> tcapi::face_area_cdf(const tissue::UVMesh& M) {
   std::vector<double> cdf(M.nf);
   double tot = 0;
   for (int f = 0; f < M.nf; ++f) {
@@ -30,22 +30,18 @@ void tcapi::random_init(const tissue::UVMesh& M, uint64_t seed, int n, float* x,
     tot += 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
     cdf[f] = tot;
   }
+  return cdf;
+}
+
+void tcapi::random_init(const tissue::UVMesh& M, uint64_t seed, int n, float* x, float* y, float* th) {
+  const std::vector<double> cdf = face_area_cdf(M);
   for (int i = 0; i < n; ++i) {
-    const tissue::Draw4 d = tissue::philox_draw(seed, (uint32_t)i, 0u, 0u);
-    const double u0 = tissue::u53(d.w[0], d.w[1]);
-    int f = (int)(std::upper_bound(cdf.begin(), cdf.end(), u0 * tot) - cdf.begin());
-    f = std::min(f, M.nf - 1);
-    const tissue::Draw4 e = tissue::philox_draw(seed, (uint32_t)i, 0u, 2u);
-    double r1 = tissue::u53(d.w[2], d.w[3]), r2 = tissue::u53(e.w[0], e.w[1]);
-    if (r1 + r2 > 1.0) { r1 = 1.0 - r1; r2 = 1.0 - r2; }
-    const double* a = &M.uv[2 * M.faces[3 * f]];
-    const double* b = &M.uv[2 * M.faces[3 * f + 1]];
-    const double* c = &M.uv[2 * M.faces[3 * f + 2]];
-    x[i] = (float)std::clamp(a[0] + r1 * (b[0] - a[0]) + r2 * (c[0] - a[0]), 0.0, 1.0);
-    y[i] = (float)std::clamp(a[1] + r1 * (b[1] - a[1]) + r2 * (c[1] - a[1]), 0.0, 1.0);
-    double t = 2.0 * M_PI * tissue::u53(e.w[2], e.w[3]);
-    t = t - 2.0 * M_PI * std::floor((t + M_PI) / (2.0 * M_PI));
-    th[i] = (float)t;
+    const tissue::InitDraw d = tissue::area_uniform_draw(seed, i, cdf.data(), M.nf);
+    const double* a = &M.uv[2 * M.faces[3 * d.face]];
+    const double* b = &M.uv[2 * M.faces[3 * d.face + 1]];
+    const double* c = &M.uv[2 * M.faces[3 * d.face + 2]];
+    tissue::barycentric_uv(a[0], a[1], b[0], b[1], c[0], c[1], d.r1, d.r2, &x[i], &y[i]);
+    th[i] = d.th;
   }
 }
 
@@ -170,6 +166,14 @@ int32_t tissue_gen_noise(uint64_t seed, double std_dev, int32_t n_steps, int32_t
     for (int t = 0; t < n_steps; ++t)
       for (int i = 0; i < n; ++i)
         out[(size_t)t * n + i] = (float)(std_dev * tissue::normal_draw(seed, (uint32_t)i, (uint32_t)t));
+    return TISSUE_OK;
+  });
+}
+
+int32_t tissue_random_init_host(const tissue_mesh* m, uint64_t seed, int32_t n, float* x, float* y, float* theta) {
+  return guarded([&] {
+    if (!m || n < 0 || (n > 0 && (!x || !y || !theta))) throw ApiError(TISSUE_E_INVALID_ARG, "bad init arguments");
+    random_init(m->m, seed, n, x, y, theta);
     return TISSUE_OK;
   });
 }

@@ -117,6 +117,9 @@ inline tdev::StepParams make_step_params(const tissue_params& p) {
 // then a uniform point of it (the UV image of a triangle is affine, so uniform in UV too);
 // headings uniform on [0, 2 pi), stored in [-pi, pi). Counter-based: particle i uses the
 // Philox4x32-10 blocks (i, 0, 0) and (i, 0, 2) of the seed.
+// cumulative FP64 3-D face areas, the face-choice table of the area-uniform initialisation
+std::vector<double> face_area_cdf(const tissue::UVMesh& M);
+// the area-uniform initialisation on the host (the GPU kernel draws the same values)
 void random_init(const tissue::UVMesh& M, uint64_t seed, int n, float* x, float* y, float* th);
 
 }  // namespace tcapi
@@ -137,6 +140,7 @@ struct tissue_mesh {
   int32_t* d_faces = nullptr;
   int32_t* d_cell_start = nullptr;
   float4* d_cell_tri = nullptr;
+  double* d_cdf = nullptr;  // cumulative face areas (area-uniform initialisation)
   int32_t* d_row_ptr = nullptr;
   int2* d_nbr = nullptr;
 
@@ -145,7 +149,7 @@ struct tissue_mesh {
     int cur = 0;
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(device);
-    dfree(d_uv); dfree(d_xyz); dfree(d_faces); dfree(d_cell_start); dfree(d_cell_tri); dfree(d_row_ptr); dfree(d_nbr);
+    dfree(d_uv); dfree(d_xyz); dfree(d_faces); dfree(d_cell_start); dfree(d_cell_tri); dfree(d_cdf); dfree(d_row_ptr); dfree(d_nbr);
     (void)hipSetDevice(cur);
     dev_ready = false;
   }
@@ -176,6 +180,8 @@ struct tissue_mesh {
       for (size_t q = 0; q < m.cell_faces.size(); ++q)
         tdev::pack_cell_triangle(reinterpret_cast<const float*>(uv.data()), m.faces.data(), m.cell_faces[q], &tri[2 * q]);
       d_cell_tri = dalloc<float4>(tri.size()); upload(d_cell_tri, tri.data(), tri.size());
+      const std::vector<double> cdf = face_area_cdf(m);
+      d_cdf = dalloc<double>(cdf.size()); upload(d_cdf, cdf.data(), cdf.size());
       d_row_ptr = dalloc<int32_t>(m.row_ptr.size()); upload(d_row_ptr, m.row_ptr.data(), m.row_ptr.size());
       d_nbr = dalloc<int2>(nbr.size()); upload(d_nbr, nbr.data(), nbr.size());
       dev_ready = true;

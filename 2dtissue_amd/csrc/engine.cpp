@@ -146,30 +146,43 @@ void Engine::set_state(const float* x, const float* y, const float* theta, const
     HIP_OK(hipMemcpyAsync(dx, x, ntot * sizeof(float), hipMemcpyHostToDevice, stream));
     HIP_OK(hipMemcpyAsync(dy, y, ntot * sizeof(float), hipMemcpyHostToDevice, stream));
     HIP_OK(hipMemcpyAsync(dt, theta, ntot * sizeof(float), hipMemcpyHostToDevice, stream));
-    const int cur = sphase;
-    HIP_OK(launch_prepare(ntot, dx, dy, dt, xycs[cur], th[cur], stream));
-    HIP_OK(launch_locate(md, ntot, xycs[cur], vid[cur], face, r3x, r3y, r3z, stream));
-    if (vid_in) HIP_OK(hipMemcpyAsync(vid[cur], vid_in, ntot * sizeof(int), hipMemcpyHostToDevice, stream));
-    HIP_OK(hipMemsetAsync(vx, 0, ntot * sizeof(float), stream));
-    HIP_OK(hipMemsetAsync(vy, 0, ntot * sizeof(float), stream));
-    HIP_OK(hipMemsetAsync(nbr, 0, ntot * sizeof(int), stream));
-    HIP_OK(hipMemsetAsync(d_halt, 0, R * sizeof(unsigned), stream));
-    HIP_OK(hipStreamSynchronize(stream));
+    adopt_state(dx, dy, dt, vid_in);
   } catch (...) {
     dfree(dx); dfree(dy); dfree(dt);
     throw;
   }
   dfree(dx); dfree(dy); dfree(dt);
+}
+
+// the state from device arrays x, y, theta (and host vertex ids, or located when null)
+void Engine::adopt_state(const float* dx, const float* dy, const float* dt, const int32_t* vid_in) {
+  const int cur = sphase;
+  HIP_OK(launch_prepare(ntot, dx, dy, dt, xycs[cur], th[cur], stream));
+  HIP_OK(launch_locate(md, ntot, xycs[cur], vid[cur], face, r3x, r3y, r3z, stream));
+  if (vid_in) HIP_OK(hipMemcpyAsync(vid[cur], vid_in, ntot * sizeof(int), hipMemcpyHostToDevice, stream));
+  HIP_OK(hipMemsetAsync(vx, 0, ntot * sizeof(float), stream));
+  HIP_OK(hipMemsetAsync(vy, 0, ntot * sizeof(float), stream));
+  HIP_OK(hipMemsetAsync(nbr, 0, ntot * sizeof(int), stream));
+  HIP_OK(hipMemsetAsync(d_halt, 0, R * sizeof(unsigned), stream));
+  HIP_OK(hipStreamSynchronize(stream));
   buckets_stale = true;
   has_state = true;
 }
 
+// area-uniform initial state drawn on the GPU (the same draws as the host's random_init)
 void Engine::init_random(const uint64_t* seeds) {
   if (!seeds) throw ApiError(TISSUE_E_INVALID_ARG, "null seeds");
-  std::vector<float> x(ntot), y(ntot), t(ntot);
-  for (int r = 0; r < R; ++r)
-    random_init(mesh->m, seeds[r], n[r], x.data() + off[r], y.data() + off[r], t.data() + off[r]);
-  set_state(x.data(), y.data(), t.data(), nullptr);
+  uint64_t* ds = dalloc<uint64_t>(R);
+  float *dx = dalloc<float>(ntot), *dy = dalloc<float>(ntot), *dt = dalloc<float>(ntot);
+  try {
+    HIP_OK(hipMemcpyAsync(ds, seeds, R * sizeof(uint64_t), hipMemcpyHostToDevice, stream));
+    HIP_OK(launch_init_random(ntot, d_rid, d_reps, ds, mesh->d_cdf, md, dx, dy, dt, stream));
+    adopt_state(dx, dy, dt, nullptr);
+  } catch (...) {
+    dfree(ds); dfree(dx); dfree(dy); dfree(dt);
+    throw;
+  }
+  dfree(ds); dfree(dx); dfree(dy); dfree(dt);
 }
 
 void Engine::get_state(float* x, float* y, float* theta, int32_t* vid_out) {

@@ -31,6 +31,7 @@ struct Engine {
   Engine& operator=(const Engine&) = delete;
 
   // state of all replicas, concatenated (replica r at off[r])
+  void adopt_state(const float* dx, const float* dy, const float* dt, const int32_t* vid_in);
   void set_state(const float* x, const float* y, const float* theta, const int32_t* vid);
   void init_random(const uint64_t* seeds);
   void get_state(float* x, float* y, float* theta, int32_t* vid);

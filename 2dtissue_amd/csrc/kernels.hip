@@ -13,6 +13,7 @@
 #include <climits>
 
 #include "launch.h"
+#include "rng.h"
 
 using namespace tdev;
 
@@ -283,6 +284,20 @@ __global__ void __launch_bounds__(256) k_seam_wrap(int n, const double2* start, 
   }
 }
 
+// area-uniform initial state of every replica (replica r from seeds[r]); the draws are those of
+// the host initialisation (rng.h), so the state equals it bit for bit
+__global__ void k_init_random(int ntot, const int* rid, const ReplicaDesc* reps, const uint64_t* seeds,
+                              const double* cdf, MeshDev mesh, float* x, float* y, float* th) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ntot) return;
+  const int r = rid[g];
+  const tissue::InitDraw d = tissue::area_uniform_draw(seeds[r], g - reps[r].off, cdf, mesh.nf);
+  const float2 a = mesh.uv[mesh.faces[3 * d.face]], b = mesh.uv[mesh.faces[3 * d.face + 1]],
+               c = mesh.uv[mesh.faces[3 * d.face + 2]];
+  tissue::barycentric_uv(a.x, a.y, b.x, b.y, c.x, c.y, d.r1, d.r2, &x[g], &y[g]);
+  th[g] = d.th;
+}
+
 __global__ void k_locate(MeshDev mesh, int n, const float4* xycs, int* vid, int* face, float* r3x, float* r3y,
                          float* r3z) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -399,6 +414,12 @@ hipError_t launch_seam_wrap(int lanes, int n, const double2* start, double2* end
     case 16: hipLaunchKernelGGL(k_seam_wrap<16>, grid, block, 0, st, n, start, end, dtheta, quarter, ok, max_crossings); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_init_random(int ntot, const int* rid, const ReplicaDesc* reps, const uint64_t* seeds,
+                              const double* cdf, const MeshDev& mesh, float* x, float* y, float* th, hipStream_t st) {
+  hipLaunchKernelGGL(k_init_random, dim3(cdiv(ntot, 256)), dim3(256), 0, st, ntot, rid, reps, seeds, cdf, mesh, x, y, th);
   return hipGetLastError();
 }
 

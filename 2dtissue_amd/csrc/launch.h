@@ -133,6 +133,9 @@ hipError_t launch_prepare(int n, const float* x, const float* y, const float* th
                           hipStream_t st);
 hipError_t launch_locate(const tdev::MeshDev& mesh, int n, const float4* xycs, int* vid, int* face, float* r3x,
                          float* r3y, float* r3z, hipStream_t st);
+// area-uniform initial positions and headings of every replica (replica r from seeds[r])
+hipError_t launch_init_random(int ntot, const int* rid, const ReplicaDesc* reps, const uint64_t* seeds,
+                              const double* cdf, const tdev::MeshDev& mesh, float* x, float* y, float* th, hipStream_t st);
 // seam wrap of n moves start -> end (end overwritten), lanes = 1, 2, 4, 8 or 16 per move
 hipError_t launch_seam_wrap(int lanes, int n, const double2* start, double2* end, double* dtheta, int* quarter, int* ok,
                             int max_crossings, hipStream_t st);

@@ -213,6 +213,10 @@ int32_t tissue_seam_wrap(int32_t n, const double* start_xy, double* end_xy, doub
                          int32_t max_crossings, int32_t lanes);
 
 /* ---- host helpers (no GPU) ---- */
+/* The area-uniform initial state that tissue_init_random(seed) draws on the GPU, computed on the
+ * host: face by 3-D area, uniform point of its UV triangle, heading uniform on [-pi, pi), from
+ * Philox4x32-10 streams of the seed (replaces the reference's unseeded draw, CellHelper.cpp:43-67). */
+int32_t tissue_random_init_host(const tissue_mesh* m, uint64_t seed, int32_t n, float* x, float* y, float* theta);
 /* eta[t][i] = std * N(0,1) from Philox4x32-10 (Salmon et al., SC'11) + Box-Muller */
 int32_t tissue_gen_noise(uint64_t seed, double std, int32_t n_steps, int32_t n, float* out);
 /* Philox4x32-10 block for (counter, key); exposed for known-answer tests */

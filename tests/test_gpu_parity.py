@@ -66,3 +66,29 @@ def test_staged_noise_rows_match_host_noise(T, mode):
         for k in ("x", "y", "theta", "vid"):
             assert np.array_equal(getattr(sa, k), getattr(sb, k)), (row, k)
     a.close(); b.close()
+
+
+def test_gpu_initialisation_matches_host(T):
+    """init_random draws the state on the GPU; it equals the host initialisation bit for bit, for a
+    single system and for every replica of a batch (replica r from its own seed)."""
+    cfg = P.C2
+    m, _ = P.mesh_and_tables(cfg["freq"], cfg["sigma"])
+    p = T.params(**dict(P.PHYS, sigma=cfg["sigma"]))
+    s = T.Sim(m, p, cfg["n"])
+    s.init_random(123)
+    st = s.get_state()
+    x, y, th = m.random_init_host(123, cfg["n"])
+    assert np.array_equal(st.x, x) and np.array_equal(st.y, y) and np.array_equal(st.theta, th)
+    s.close()
+    sizes = [100, 2500, 731]
+    b = T.Batch(m, sizes, [p] * len(sizes))
+    seeds = np.array([5, 6, 2**40 + 3], np.uint64)
+    b.init_random(seeds)
+    bs = b.get_state()
+    o = 0
+    for r, nr in enumerate(sizes):
+        x, y, th = m.random_init_host(int(seeds[r]), nr)
+        assert np.array_equal(bs.x[o:o + nr], x) and np.array_equal(bs.y[o:o + nr], y), r
+        assert np.array_equal(bs.theta[o:o + nr], th), r
+        o += nr
+    b.close()

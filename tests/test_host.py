@@ -176,3 +176,62 @@ def test_mesh_errors(T):
         T.Mesh.ellipsoid(-1.0, 1.0, 1.0, 4)
     with pytest.raises(T.TissueError):
         T.Mesh.load_off("/nonexistent/mesh.off")
+
+
+def restated_init(T, tb, seed, n):
+    """The area-uniform initialisation restated in numpy from its definition: cumulative FP64 3-D
+    face areas, upper-bound face choice, folded barycentrics, heading 2 pi u wrapped to [-pi, pi),
+    all from the Philox4x32-10 block (pinned by its known answers above)."""
+    xyz, faces, uv = np.asarray(tb["xyz"]), np.asarray(tb["faces"]), np.asarray(tb["uv"])
+    A, B, Cc = xyz[faces[:, 0]], xyz[faces[:, 1]], xyz[faces[:, 2]]
+    e1, e2 = B - A, Cc - A
+    cx = e1[:, 1] * e2[:, 2] - e1[:, 2] * e2[:, 1]
+    cy = e1[:, 2] * e2[:, 0] - e1[:, 0] * e2[:, 2]
+    cz = e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0]
+    area = 0.5 * np.sqrt(cx * cx + cy * cy + cz * cz)
+    cdf = np.cumsum(area)
+    u53 = lambda a, b: ((a >> 5) * 67108864.0 + (b >> 6)) / 9007199254740992.0
+    key = [seed & 0xFFFFFFFF, seed >> 32]
+    x = np.zeros(n, np.float32); y = np.zeros(n, np.float32); th = np.zeros(n, np.float32)
+    for i in range(n):
+        d = T.philox4x32([i, 0, 0, 0], key)
+        e = T.philox4x32([i, 0, 2, 0], key)
+        f = min(int(np.searchsorted(cdf, u53(d[0], d[1]) * cdf[-1], side="right")), len(faces) - 1)
+        r1, r2 = u53(d[2], d[3]), u53(e[0], e[1])
+        if r1 + r2 > 1.0:
+            r1, r2 = 1.0 - r1, 1.0 - r2
+        a, b, c = uv[faces[f, 0]], uv[faces[f, 1]], uv[faces[f, 2]]
+        px = a[0] + r1 * (b[0] - a[0]) + r2 * (c[0] - a[0])
+        py = a[1] + r1 * (b[1] - a[1]) + r2 * (c[1] - a[1])
+        x[i] = np.float32(min(max(px, 0.0), 1.0)); y[i] = np.float32(min(max(py, 0.0), 1.0))
+        t = 2.0 * math.pi * u53(e[2], e[3])
+        th[i] = np.float32(t - 2.0 * math.pi * math.floor((t + math.pi) / (2.0 * math.pi)))
+    return x, y, th
+
+
+def test_random_init_host_matches_definition(T, ell):
+    m, tb = ell
+    x, y, th = m.random_init_host(7, 400)
+    rx, ry, rth = restated_init(T, tb, 7, 400)
+    assert np.array_equal(x, rx) and np.array_equal(y, ry) and np.array_equal(th, rth)
+    x2, _, _ = m.random_init_host(8, 400)
+    assert not np.array_equal(x, x2)
+
+
+def test_random_init_host_is_area_uniform(T, O, ell):
+    """Faces are chosen with probability proportional to their 3-D area: the share of particles
+    whose remapped point lies on faces with centroid z > 0 matches those faces' share of the area."""
+    m, tb = ell
+    n = 20000
+    x, y, th = m.random_init_host(3, n)
+    assert ((x >= 0) & (x <= 1) & (y >= 0) & (y <= 1)).all()
+    assert ((th >= -np.pi) & (th < np.pi)).all()
+    assert abs(float(th.mean())) < 0.05
+    _, _, face = O.remap(O.Mesh(tb), x.astype(np.float64), y.astype(np.float64))
+    xyz, faces = np.asarray(tb["xyz"]), np.asarray(tb["faces"])
+    A, B, Cc = xyz[faces[:, 0]], xyz[faces[:, 1]], xyz[faces[:, 2]]
+    area = 0.5 * np.linalg.norm(np.cross(B - A, Cc - A), axis=1)
+    upper = (A + B + Cc)[:, 2] > 0
+    share = area[upper].sum() / area.sum()
+    got = upper[face].mean()
+    assert abs(got - share) < 4 * np.sqrt(share * (1 - share) / n), (got, share)
