@@ -162,6 +162,8 @@ SIGNATURES = [
     ("tissue_philox4x32", None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     ("tissue_seam_wrap", C.c_int32, [C.c_int32, _D, _D, _D, _I, C.c_int32, C.c_int32]),
     ("tissue_random_init_host", C.c_int32, [_P, C.c_uint64, C.c_int32, _F, _F, _F]),
+    ("tissue_record_start", C.c_int32, [_P, C.c_char_p, C.c_int32]),
+    ("tissue_record_stop", C.c_int32, [_P]),
 ]
 
 
@@ -370,6 +372,13 @@ class Sim:
 
     def set_launch(self, lanes: int = 0, block: int = 0) -> None:
         _check(lib().tissue_set_launch(self.h, lanes, block))
+
+    def record_start(self, directory: str, every: int = 1) -> None:
+        """Per-step CSV output stream (tissue_record_start) into an existing directory."""
+        _check(lib().tissue_record_start(self.h, os.fsencode(directory), every))
+
+    def record_stop(self) -> None:
+        _check(lib().tissue_record_stop(self.h))
 
     def set_mode(self, mode: int) -> None:
         _check(lib().tissue_set_mode(self.h, mode))

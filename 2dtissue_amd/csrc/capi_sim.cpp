@@ -88,6 +88,17 @@ int32_t tissue_stage_noise(tissue_sim* s, const float* noise, int32_t n_rows) {
   });
 }
 
+int32_t tissue_record_start(tissue_sim* s, const char* dir, int32_t every) {
+  return guarded([&] {
+    eng(s).record_start(dir, every);
+    return TISSUE_OK;
+  });
+}
+
+int32_t tissue_record_stop(tissue_sim* s) {
+  return guarded([&] { return eng(s).record_stop(); });
+}
+
 int32_t tissue_step_staged(tissue_sim* s, int32_t n_steps, int32_t first_row) {
   return guarded([&] {
     auto& e = eng(s);

@@ -7,6 +7,7 @@
 #include <numeric>
 #include <thread>
 
+#include "record.h"
 #include "rng.h"
 
 namespace tissue {
@@ -65,9 +66,9 @@ Engine::Engine(tissue_mesh* m, int R_, const int* n_, const tissue_params* p_) {
     HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIP_OK(hipEventCreate(&ev0));
     HIP_OK(hipEventCreate(&ev1));
-    d_row0 = dalloc<int>(1);
-    HIP_OK(hipHostMalloc((void**)&h_row0, sizeof(int), hipHostMallocDefault));
-    *h_row0 = 0;
+    d_row0 = dalloc<int>(2);
+    HIP_OK(hipHostMalloc((void**)&h_row0, 2 * sizeof(int), hipHostMallocDefault));
+    h_row0[0] = h_row0[1] = 0;
     if (nmax <= resident_cap) HIP_OK(resident_prepare(md.nv, nmax));
     d_reps = dalloc<ReplicaDesc>(R);
     upload(d_reps, reps.data(), reps.size());
@@ -108,6 +109,12 @@ void Engine::drop_graphs() {
 void Engine::release() {
   (void)hipSetDevice(device);
   if (stream) (void)hipStreamSynchronize(stream);
+  if (rec) {  // finish the files; an I/O error cannot be reported from here
+    rec->stop();
+    dfree(rec->d_ring);
+    dfree(rec->d_ring_nbr);
+    rec.reset();
+  }
   drop_graphs();
   dfree(d_reps); dfree(d_rid);
   for (int k = 0; k < 2; ++k) { dfree(xycs[k]); dfree(th[k]); dfree(vid[k]); }
@@ -458,6 +465,9 @@ void Engine::enqueue(const Shape& s, int n_steps, const float* nz, size_t stride
         io.step = t;
         HIP_OK(launch_step_tiled(s.lanes, s.block, false, md, d_reps, d_tiles, n_phys, bs[bp], bs[(bp + 1) % 3],
                                  bs[(bp + 2) % 3], io, stream));
+        if (rec)
+          HIP_OK(launch_record(ntot, d_row0, t, rec->every, rec->depth, io.xycs_out, r3x, r3y, r3z, nbr, rec->d_ring,
+                               rec->d_ring_nbr, stream));
         HIP_OK(launch_bucket_finalize(s.block, d_reps, d_tiles, n_phys, md.nv, bs[(bp + 1) % 3], stream));
         sp = 1 - sp;
         bp = (bp + 1) % 3;
@@ -469,10 +479,29 @@ void Engine::enqueue(const Shape& s, int n_steps, const float* nz, size_t stride
 }
 
 int32_t Engine::run(int n_steps, const float* nz, size_t stride, int row0) {
-  return run_shape(resolve(), n_steps, nz, stride, row0);
+  Shape s = resolve();
+  if (rec && s.mode == TISSUE_MODE_RESIDENT) s = resolve_tiled();  // the stream records every step
+  return run_shape(s, n_steps, nz, stride, row0);
 }
 
 int32_t Engine::run_shape(const Shape& s, int n_steps, const float* nz, size_t stride, int row0) {
+  if (!rec) return run_once(s, n_steps, nz, stride, row0);
+  if (s.mode != TISSUE_MODE_STEP_KERNEL)
+    throw ApiError(TISSUE_E_INVALID_ARG, "the per-step output stream needs the per-step kernel");
+  // runs short enough that no ring slot is overwritten before it is copied out
+  const int chunk = rec->depth * rec->every;
+  int32_t status = TISSUE_OK;
+  for (int done = 0; done < n_steps; done += chunk) {
+    const int c = std::min(chunk, n_steps - done);
+    const long base = steps_done - rec->origin;
+    const int32_t st = run_once(s, c, nz, stride, row0 + done);
+    record_collect(base, c);
+    if (status == TISSUE_OK) status = st;
+  }
+  return status;
+}
+
+int32_t Engine::run_once(const Shape& s, int n_steps, const float* nz, size_t stride, int row0) {
   if (!has_state) throw ApiError(TISSUE_E_INVALID_ARG, "no state: call init_random or set_state first");
   if (n_steps < 0) throw ApiError(TISSUE_E_INVALID_ARG, "negative step count");
   if (n_steps == 0) return TISSUE_OK;
@@ -520,8 +549,10 @@ int32_t Engine::run_shape(const Shape& s, int n_steps, const float* nz, size_t s
     HIP_OK(e);
     it = graphs.emplace(key, ge).first;
   }
-  *h_row0 = nz ? row0 : 0;  // the previous run has completed (synchronised below), so the pinned word is free
-  HIP_OK(hipMemcpyAsync(d_row0, h_row0, sizeof(int), hipMemcpyHostToDevice, stream));
+  // the previous run has completed (synchronised below), so the pinned words are free
+  h_row0[0] = nz ? row0 : 0;
+  h_row0[1] = (int)std::min<long>(steps_done - (rec ? rec->origin : 0), INT_MAX);
+  HIP_OK(hipMemcpyAsync(d_row0, h_row0, 2 * sizeof(int), hipMemcpyHostToDevice, stream));
   HIP_OK(hipEventRecord(ev0, stream));
   HIP_OK(hipGraphLaunch(it->second, stream));
   HIP_OK(hipEventRecord(ev1, stream));
@@ -568,6 +599,7 @@ int32_t Engine::run_shape(const Shape& s, int n_steps, const float* nz, size_t s
 
 Shape Engine::autotune(int n_steps) {
   if (!has_state) throw ApiError(TISSUE_E_INVALID_ARG, "no state to tune on");
+  if (rec) throw ApiError(TISSUE_E_INVALID_ARG, "stop the output stream before autotuning");
   if (n_steps < 1) throw ApiError(TISSUE_E_INVALID_ARG, "autotune needs at least one step");
   std::vector<Shape> cands;
   for (int l : {1, 2, 4, 8})

@@ -53,6 +53,8 @@ struct Engine {
   // captured step graphs do not depend on it (it is read from device memory)
   int32_t run(int n_steps, const float* noise, size_t stride, int row0 = 0);
   int32_t run_shape(const Shape& s, int n_steps, const float* noise, size_t stride, int row0 = 0);
+  int32_t run_once(const Shape& s, int n_steps, const float* noise, size_t stride, int row0);
+  void record_collect(long base, int n_steps);
   Shape resolve() const;
   // the request resolved for the tiled step kernel (the adjoint records its tape there)
   Shape resolve_tiled() const;
@@ -76,8 +78,14 @@ struct Engine {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int* d_row0 = nullptr;  // first noise row of the current run, read by the step kernels
-  int* h_row0 = nullptr;  // pinned host copy it is uploaded from
+  // control words of the current run, read by its kernels: [0] first noise row, [1] steps done
+  // before the run (numbering of recorded steps); uploaded from pinned host memory
+  int* d_row0 = nullptr;
+  int* h_row0 = nullptr;
+  // per-step output stream (engine_record.cpp); null when not recording
+  std::unique_ptr<struct Recorder> rec;
+  void record_start(const char* dir, int every);
+  int32_t record_stop();
   int R = 0, ntot = 0, nmax = 0, resident_cap = 0;
   std::vector<int> n, off;
   std::vector<tissue_params> p;

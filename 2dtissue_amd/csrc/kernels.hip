@@ -298,6 +298,25 @@ __global__ void k_init_random(int ntot, const int* rid, const ReplicaDesc* reps,
   th[g] = d.th;
 }
 
+// per-step output stream: copies step t's UV position, 3-D position and neighbour count into the
+// ring slot of its global step number g = ctl[1] + t + 1 when g is a multiple of `every`
+__global__ void k_record(int ntot, const int* ctl, int t, int every, int depth, const float4* xycs, const float* r3x,
+                         const float* r3y, const float* r3z, const int* nbr, float* ring, int* ring_nbr) {
+  const long g = (long)ctl[1] + t + 1;
+  if (g % every != 0) return;  // uniform over the grid
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ntot) return;
+  const size_t slot = (size_t)((g / every) % depth);
+  float* f = ring + slot * 5 * (size_t)ntot;
+  const float4 q = xycs[i];
+  f[i] = q.x;
+  f[(size_t)ntot + i] = q.y;
+  f[2 * (size_t)ntot + i] = r3x[i];
+  f[3 * (size_t)ntot + i] = r3y[i];
+  f[4 * (size_t)ntot + i] = r3z[i];
+  ring_nbr[slot * (size_t)ntot + i] = nbr[i];
+}
+
 __global__ void k_locate(MeshDev mesh, int n, const float4* xycs, int* vid, int* face, float* r3x, float* r3y,
                          float* r3z) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -420,6 +439,13 @@ hipError_t launch_seam_wrap(int lanes, int n, const double2* start, double2* end
 hipError_t launch_init_random(int ntot, const int* rid, const ReplicaDesc* reps, const uint64_t* seeds,
                               const double* cdf, const MeshDev& mesh, float* x, float* y, float* th, hipStream_t st) {
   hipLaunchKernelGGL(k_init_random, dim3(cdiv(ntot, 256)), dim3(256), 0, st, ntot, rid, reps, seeds, cdf, mesh, x, y, th);
+  return hipGetLastError();
+}
+
+hipError_t launch_record(int ntot, const int* ctl, int t, int every, int depth, const float4* xycs, const float* r3x,
+                         const float* r3y, const float* r3z, const int* nbr, float* ring, int* ring_nbr, hipStream_t st) {
+  hipLaunchKernelGGL(k_record, dim3(cdiv(ntot, 256)), dim3(256), 0, st, ntot, ctl, t, every, depth, xycs, r3x, r3y,
+                     r3z, nbr, ring, ring_nbr);
   return hipGetLastError();
 }
 

@@ -133,6 +133,10 @@ hipError_t launch_prepare(int n, const float* x, const float* y, const float* th
                           hipStream_t st);
 hipError_t launch_locate(const tdev::MeshDev& mesh, int n, const float4* xycs, int* vid, int* face, float* r3x,
                          float* r3y, float* r3z, hipStream_t st);
+// copy step t's outputs into the per-step output ring (when its global step number, ctl[1] + t + 1,
+// is a multiple of every): ring [depth][5][ntot] = x, y, X, Y, Z; ring_nbr [depth][ntot]
+hipError_t launch_record(int ntot, const int* ctl, int t, int every, int depth, const float4* xycs, const float* r3x,
+                         const float* r3y, const float* r3z, const int* nbr, float* ring, int* ring_nbr, hipStream_t st);
 // area-uniform initial positions and headings of every replica (replica r from seeds[r])
 hipError_t launch_init_random(int ntot, const int* rid, const ReplicaDesc* reps, const uint64_t* seeds,
                               const double* cdf, const tdev::MeshDev& mesh, float* x, float* y, float* th, hipStream_t st);

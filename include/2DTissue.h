@@ -24,12 +24,11 @@
 //     from the mesh itself (the reference reads it from MeshCartographyLib's <stem>_open.off).
 // With save_data the per-step CSV files of the reference (2DTissue.cpp:270-280, IO.h:40-69) are
 // appended to data_dir (default "data"): r_data_<step>.csv, r_data_3D_<step>.csv,
-// particles_color_<step>.csv, values written with 15 significant digits.
+// particles_color_<step>.csv, values written with 15 significant digits, by the library's output
+// stream (tissue_record_start) in the background; they are complete once the last step has run.
 #pragma once
 
 #include <cmath>
-#include <fstream>
-#include <iomanip>
 #include <memory>
 #include <sstream>
 #include <stdexcept>
@@ -84,6 +83,7 @@ class _2DTissue {
   }
 
   ~_2DTissue() {
+    if (sim_) (void)tissue_record_stop(sim_);
     if (sim_) tissue_destroy(sim_);
     if (mesh_) tissue_mesh_destroy(mesh_);
   }
@@ -93,6 +93,8 @@ class _2DTissue {
   // 2DTissue.cpp:117-134
   void start() {
     check(tissue_init_random(sim_, seed_));
+    // the library's per-step output stream writes the reference's CSV files in the background
+    if (save_data_) check(tissue_record_start(sim_, data_dir_.c_str(), 1));
     current_step_ = 0;
     finished_ = step_count_ == 0;
     noise_rows_.clear();
@@ -143,8 +145,10 @@ class _2DTissue {
     // the reference reports the last slot of v_order, which fills on the final step (2DTissue.cpp:165)
     system.order_parameter = v_order_.empty() ? 0.0 : v_order_.back();
     ++current_step_;
-    if (current_step_ >= step_count_) finished_ = true;
-    if (save_data_) save_our_data(system);
+    if (current_step_ >= step_count_) {
+      finished_ = true;
+      if (save_data_) check(tissue_record_stop(sim_));  // every CSV file complete
+    }
     return system;
   }
 
@@ -182,22 +186,6 @@ class _2DTissue {
       return tissue_mesh_perturbed_sphere(v[0], (int32_t)v[1], out);
     }
     return tissue_mesh_load_off(path.c_str(), out);
-  }
-
-  // 2DTissue.cpp:270-280 with IO.h:40-69: one row per particle, appended
-  void save_our_data(const System& s) const {
-    const std::string step = std::to_string(current_step_);
-    std::ofstream a(data_dir_ + "/r_data_" + step + ".csv", std::ios::app);
-    std::ofstream b(data_dir_ + "/r_data_3D_" + step + ".csv", std::ios::app);
-    std::ofstream c(data_dir_ + "/particles_color_" + step + ".csv", std::ios::app);
-    if (!a || !b || !c) throw std::runtime_error("cannot open the CSV files in " + data_dir_);
-    a << std::setprecision(15);
-    b << std::setprecision(15);
-    for (const Particle& p : s.particles) {
-      a << p.x_UV << "," << p.y_UV << "\n";
-      b << p.x_3D << "," << p.y_3D << "," << p.z_3D << "\n";
-      c << p.neighbor_count << "\n";
-    }
   }
 
   bool save_data_;

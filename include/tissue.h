@@ -200,6 +200,19 @@ int32_t tissue_adjoint(tissue_sim* s, int32_t n_steps, int32_t first_row, int32_
 int32_t tissue_batch_adjoint(tissue_batch* b, int32_t n_steps, int32_t first_row, int32_t ckpt_every,
                              const tissue_loss* loss, tissue_grads* g, double* d_theta0, double* d_x0, double* d_y0);
 
+/* ---- per-step output stream ---- */
+/* From the next step on, every `every`-th step's UV positions, 3-D positions and neighbour counts
+ * are written to the reference's per-step CSV files in dir (which must exist):
+ * r_data_<step>.csv (x,y), r_data_3D_<step>.csv (x,y,z), particles_color_<step>.csv (count),
+ * appended, 15 significant digits, <step> = steps done since this call (1 after the first); replaces the
+ * per-step writes of _2DTissue::update (2DTissue.cpp:168-184,270-280, IO.h:40-69). The outputs are
+ * copied on the GPU into a ring, to the host after each run, and written by a background thread
+ * while the GPU runs on; runs use the per-step kernel while recording. */
+int32_t tissue_record_start(tissue_sim* s, const char* dir, int32_t every);
+/* finishes writing every recorded step and stops the stream (TISSUE_E_IO if a write failed);
+ * destroying the simulation also finishes the files */
+int32_t tissue_record_stop(tissue_sim* s);
+
 /* ---- seam wrap on its own ---- */
 /* Applies the seam wrap of the step to n moves on the GPU, as
  * EuclideanTiling::diagonal_seam_edges_square_border (EuclideanTiling.cpp:31-69) does to r_UV_old
