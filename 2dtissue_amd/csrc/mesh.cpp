@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <fstream>
@@ -13,6 +14,7 @@
 #include <queue>
 #include <sstream>
 #include <stdexcept>
+#include <thread>
 #include <unordered_map>
 
 namespace tissue {
@@ -549,6 +551,12 @@ double unfold_across(const double* V, const double* A, const double* B, const do
 
 }  // namespace
 
+// worker threads of the table build: the host's cores, at most 16
+int build_threads() {
+  const unsigned h = std::thread::hardware_concurrency();
+  return (int)std::clamp<unsigned>(h == 0 ? 1u : h, 1u, 16u);
+}
+
 void build_distance_table(UVMesh& m, double cutoff) {
   if (!(cutoff > 0)) throw std::runtime_error("distance table: cutoff must be positive");
   const int nv = m.nv;
@@ -561,82 +569,117 @@ void build_distance_table(UVMesh& m, double cutoff) {
   auto edge_key = [](int a, int b) { return a < b ? ((int64_t)a << 32) | b : ((int64_t)b << 32) | a; };
   for (int f = 0; f < m.nf; ++f)
     for (int q = 0; q < 3; ++q) ef[edge_key(m.faces[3 * f + q], m.faces[3 * f + (q + 1) % 3])].push_back(f);
-  std::vector<double> T(nv, std::numeric_limits<double>::infinity());
-  std::vector<char> alive(nv, 0);
-  std::vector<int> touched;
   std::vector<std::vector<std::pair<int, double>>> rows(nv);
   using QE = std::pair<double, int>;
-  for (int s = 0; s < nv; ++s) {
-    std::priority_queue<QE, std::vector<QE>, std::greater<QE>> pq;
-    auto relax = [&](int tgt, double cand) {
-      if (cand < T[tgt]) {
-        if (T[tgt] == std::numeric_limits<double>::infinity()) touched.push_back(tgt);
-        T[tgt] = cand;
-        pq.push({cand, tgt});
-      }
-    };
-    T[s] = 0;
-    touched.push_back(s);
-    pq.push({0.0, s});
-    while (!pq.empty()) {
-      auto [tv, v] = pq.top();
-      pq.pop();
-      if (alive[v] || tv > T[v]) continue;
-      if (tv > lim) break;
-      alive[v] = 1;
-      rows[s].push_back({v, tv});
-      const double* X = &m.xyz[3 * v];
-      for (int f : vf[v]) {
-        const int32_t* fc = &m.faces[3 * f];
-        for (int q = 0; q < 3; ++q) {
-          const int tgt = fc[q];
-          if (tgt == v || alive[tgt]) continue;
-          const int oth = fc[0] + fc[1] + fc[2] - v - tgt;
-          double d[3];
-          sub3(&m.xyz[3 * tgt], X, d);
-          double cand = tv + norm3(d);
-          if (alive[oth]) cand = std::min(cand, tri_update(X, tv, &m.xyz[3 * oth], T[oth], &m.xyz[3 * tgt]));
-          relax(tgt, cand);
+  // one local march per source vertex; sources are independent, so threads take them in turn,
+  // each with its own arrival times (the rows come out the same whatever the thread count)
+  std::atomic<int> next_source{0};
+  auto march = [&]() {
+    std::vector<double> T(nv, std::numeric_limits<double>::infinity());
+    std::vector<char> alive(nv, 0);
+    std::vector<int> touched;
+    for (int s = next_source++; s < nv; s = next_source++) {
+      std::priority_queue<QE, std::vector<QE>, std::greater<QE>> pq;
+      auto relax = [&](int tgt, double cand) {
+        if (cand < T[tgt]) {
+          if (T[tgt] == std::numeric_limits<double>::infinity()) touched.push_back(tgt);
+          T[tgt] = cand;
+          pq.push({cand, tgt});
         }
-        // straight segment from v across the opposite edge (a, b) into the neighbouring face
-        const int a = fc[0] == v ? fc[1] : fc[0];
-        const int b = fc[0] + fc[1] + fc[2] - v - a;
-        for (int g2 : ef[edge_key(a, b)]) {
-          if (g2 == f) continue;
-          const int32_t* gc = &m.faces[3 * g2];
-          const int x = gc[0] + gc[1] + gc[2] - a - b;
-          if (alive[x]) continue;
-          double L = unfold_across(X, &m.xyz[3 * a], &m.xyz[3 * b], &m.xyz[3 * x]);
-          relax(x, tv + L);
+      };
+      T[s] = 0;
+      touched.push_back(s);
+      pq.push({0.0, s});
+      while (!pq.empty()) {
+        auto [tv, v] = pq.top();
+        pq.pop();
+        if (alive[v] || tv > T[v]) continue;
+        if (tv > lim) break;
+        alive[v] = 1;
+        rows[s].push_back({v, tv});
+        const double* X = &m.xyz[3 * v];
+        for (int f : vf[v]) {
+          const int32_t* fc = &m.faces[3 * f];
+          for (int q = 0; q < 3; ++q) {
+            const int tgt = fc[q];
+            if (tgt == v || alive[tgt]) continue;
+            const int oth = fc[0] + fc[1] + fc[2] - v - tgt;
+            double d[3];
+            sub3(&m.xyz[3 * tgt], X, d);
+            double cand = tv + norm3(d);
+            if (alive[oth]) cand = std::min(cand, tri_update(X, tv, &m.xyz[3 * oth], T[oth], &m.xyz[3 * tgt]));
+            relax(tgt, cand);
+          }
+          // straight segment from v across the opposite edge (a, b) into the neighbouring face
+          const int a = fc[0] == v ? fc[1] : fc[0];
+          const int b = fc[0] + fc[1] + fc[2] - v - a;
+          for (int g2 : ef.at(edge_key(a, b))) {
+            if (g2 == f) continue;
+            const int32_t* gc = &m.faces[3 * g2];
+            const int x = gc[0] + gc[1] + gc[2] - a - b;
+            if (alive[x]) continue;
+            double L = unfold_across(X, &m.xyz[3 * a], &m.xyz[3 * b], &m.xyz[3 * x]);
+            relax(x, tv + L);
+          }
         }
       }
+      for (int v : touched) { T[v] = std::numeric_limits<double>::infinity(); alive[v] = 0; }
+      touched.clear();
     }
-    for (int v : touched) { T[v] = std::numeric_limits<double>::infinity(); alive[v] = 0; }
-    touched.clear();
+  };
+  {
+    std::vector<std::thread> pool;
+    const int nt = build_threads();
+    for (int k = 1; k < nt; ++k) pool.emplace_back(march);
+    march();
+    for (auto& t : pool) t.join();
   }
   // symmetrize with the minimum of both directions (transform_into_symmetric_matrix,
-  // src/simulation/Locomotion.cpp:127-138) and round to FP32
-  std::vector<std::unordered_map<int, double>> sym(nv);
+  // src/simulation/Locomotion.cpp:127-138) and round to FP32: row v gathers its own entries and
+  // the transposed entries of the rows that reach v, keeps the minimum per column, and sorts by
+  // (FP32 distance, column)
+  std::vector<int32_t> in_ptr(nv + 1, 0);
   for (int s = 0; s < nv; ++s)
-    for (auto [v, t] : rows[s]) {
-      auto upd = [&](int a, int b) {
-        auto it = sym[a].find(b);
-        if (it == sym[a].end()) sym[a].emplace(b, t);
-        else it->second = std::min(it->second, t);
-      };
-      upd(s, v);
-      upd(v, s);
+    for (auto [v, t] : rows[s]) ++in_ptr[v + 1];
+  for (int v = 0; v < nv; ++v) in_ptr[v + 1] += in_ptr[v];
+  std::vector<std::pair<int, double>> in(in_ptr[nv]);
+  {
+    std::vector<int32_t> fill(in_ptr.begin(), in_ptr.end() - 1);
+    for (int s = 0; s < nv; ++s)
+      for (auto [v, t] : rows[s]) in[fill[v]++] = {s, t};
+  }
+  std::vector<std::vector<std::pair<float, int>>> out(nv);
+  std::atomic<int> next_row{0};
+  auto merge = [&]() {
+    std::vector<std::pair<int, double>> r;
+    for (int v = next_row++; v < nv; v = next_row++) {
+      r.assign(rows[v].begin(), rows[v].end());
+      r.insert(r.end(), in.begin() + in_ptr[v], in.begin() + in_ptr[v + 1]);
+      std::sort(r.begin(), r.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+      auto& o = out[v];
+      o.clear();
+      for (size_t k = 0; k < r.size();) {
+        const int w = r[k].first;
+        double t = r[k].second;
+        for (++k; k < r.size() && r[k].first == w; ++k) t = std::min(t, r[k].second);
+        o.push_back({(float)(w == v ? 0.0 : t), w});
+      }
+      std::sort(o.begin(), o.end());
     }
+  };
+  {
+    std::vector<std::thread> pool;
+    const int nt = build_threads();
+    for (int k = 1; k < nt; ++k) pool.emplace_back(merge);
+    merge();
+    for (auto& t : pool) t.join();
+  }
   m.cutoff = cutoff;
   m.row_ptr.assign(nv + 1, 0);
   m.col.clear();
   m.dist.clear();
   for (int v = 0; v < nv; ++v) {
-    std::vector<std::pair<float, int>> r;
-    r.reserve(sym[v].size());
-    for (auto [w, t] : sym[v]) r.push_back({(float)(w == v ? 0.0 : t), w});
-    std::sort(r.begin(), r.end());
-    for (auto [d, w] : r) { m.col.push_back(w); m.dist.push_back(d); }
+    for (auto [d, w] : out[v]) { m.col.push_back(w); m.dist.push_back(d); }
     m.row_ptr[v + 1] = (int32_t)m.col.size();
   }
 }
