@@ -120,10 +120,7 @@ void Engine::release() {
   for (int k = 0; k < 2; ++k) { dfree(xycs[k]); dfree(th[k]); dfree(vid[k]); }
   dfree(vx); dfree(vy); dfree(r3x); dfree(r3y); dfree(r3z); dfree(face); dfree(nbr);
   dfree(d_halt); dfree(d_rows); dfree(d_first_bad);
-  for (int k = 0; k < 3; ++k) {
-    dfree(bs[k].count); dfree(bs[k].slot); dfree(bs[k].ovf_n); dfree(bs[k].ovf);
-    dfree(bs[k].multi_n); dfree(bs[k].multi); dfree(bs[k].big_n); dfree(bs[k].big);
-  }
+  for (int k = 0; k < 3; ++k) free_bucket_set(bs[k]);
   dfree(d_tiles); dfree(d_tile_begin); dfree(partial); dfree(phi); dfree(noise); dfree(call_noise); dfree(d_row0);
   if (h_row0) (void)hipHostFree(h_row0);
   h_row0 = nullptr;
@@ -223,34 +220,53 @@ void Engine::get_outputs(float* ovx, float* ovy, float* r3d, int32_t* oface, int
 
 // ------------------------------------------------------------------ buckets and tiles
 
+void Engine::alloc_bucket_set(BucketSet& b) {
+  const size_t cells = (size_t)R * md.nv;
+  b.count = dalloc<int>(cells);
+  b.slot = dalloc<int>(cells * tdev::kSlots);
+  b.ovf_n = dalloc<int>(R);
+  b.ovf = dalloc<int2>(ntot);
+  b.multi_n = dalloc<int>(R);
+  b.multi = dalloc<int>(ntot);
+  b.big_n = dalloc<int>(R);
+  b.big = dalloc<int>(ntot);
+}
+
+void Engine::free_bucket_set(BucketSet& b) {
+  dfree(b.count); dfree(b.slot); dfree(b.ovf_n); dfree(b.ovf);
+  dfree(b.multi_n); dfree(b.multi); dfree(b.big_n); dfree(b.big);
+}
+
+size_t Engine::bucket_set_bytes() const {
+  return (size_t)R * md.nv * (1 + tdev::kSlots) * sizeof(int) + (size_t)ntot * (sizeof(int2) + 2 * sizeof(int)) +
+         3 * (size_t)R * sizeof(int);
+}
+
+// outside any capture: clear set b (count and list lengths)
+void Engine::clear_bucket_set(BucketSet& b) {
+  HIP_OK(hipMemsetAsync(b.count, 0, (size_t)R * md.nv * sizeof(int), stream));
+  HIP_OK(hipMemsetAsync(b.ovf_n, 0, R * sizeof(int), stream));
+  HIP_OK(hipMemsetAsync(b.multi_n, 0, R * sizeof(int), stream));
+  HIP_OK(hipMemsetAsync(b.big_n, 0, R * sizeof(int), stream));
+}
+
+// outside any capture: clear set b, bucket the vertex ids v into it and sort its buckets
+void Engine::bucket_state_into(const int* v, BucketSet& b) {
+  if (!d_tiles) ensure_tiles(resolve_tiled());
+  clear_bucket_set(b);
+  HIP_OK(launch_insert(ntot, v, d_rid, d_reps, md.nv, b, stream));
+  HIP_OK(launch_bucket_finalize(std::get<1>(tiles_key), d_reps, d_tiles, n_phys, md.nv, b, stream));
+}
+
 void Engine::ensure_buckets() {
   if (buckets_ready) return;
-  const size_t cells = (size_t)R * md.nv;
-  for (int k = 0; k < 3; ++k) {
-    bs[k].count = dalloc<int>(cells);
-    bs[k].slot = dalloc<int>(cells * tdev::kSlots);
-    bs[k].ovf_n = dalloc<int>(R);
-    bs[k].ovf = dalloc<int2>(ntot);
-    bs[k].multi_n = dalloc<int>(R);
-    bs[k].multi = dalloc<int>(ntot);
-    bs[k].big_n = dalloc<int>(R);
-    bs[k].big = dalloc<int>(ntot);
-  }
+  for (int k = 0; k < 3; ++k) alloc_bucket_set(bs[k]);
   buckets_ready = true;
   buckets_stale = true;
 }
 
 // outside any capture: clear set k, bucket the vertex ids v into it and sort its buckets
-void Engine::bucket_state(const int* v, int k) {
-  if (!d_tiles) ensure_tiles(resolve_tiled());
-  const size_t cells = (size_t)R * md.nv;
-  HIP_OK(hipMemsetAsync(bs[k].count, 0, cells * sizeof(int), stream));
-  HIP_OK(hipMemsetAsync(bs[k].ovf_n, 0, R * sizeof(int), stream));
-  HIP_OK(hipMemsetAsync(bs[k].multi_n, 0, R * sizeof(int), stream));
-  HIP_OK(hipMemsetAsync(bs[k].big_n, 0, R * sizeof(int), stream));
-  HIP_OK(launch_insert(ntot, v, d_rid, d_reps, md.nv, bs[k], stream));
-  HIP_OK(launch_bucket_finalize(std::get<1>(tiles_key), d_reps, d_tiles, n_phys, md.nv, bs[k], stream));
-}
+void Engine::bucket_state(const int* v, int k) { bucket_state_into(v, bs[k]); }
 
 // outside any capture: clear every set, bucket the current state into set bphase
 void Engine::rebuild_buckets() {

@@ -139,6 +139,8 @@ struct Engine {
     unsigned* rows = nullptr; int* first_bad = nullptr;
     int* sexp = nullptr;  // per replica: the adjoint state is stored scaled by 2^-sexp
     OrderPartial* partial = nullptr; int partial_n = 0;
+    // bucket sets of every state of a segment (kept for the reverse steps), when they fit
+    std::vector<BucketSet> sbs;
   } adj;
   void adjoint_buffers(int K, int C);
   void adjoint_release();
@@ -148,6 +150,11 @@ struct Engine {
   void ensure_buckets();
   void rebuild_buckets();
   void bucket_state(const int* vid, int set);
+  void bucket_state_into(const int* vid, BucketSet& b);
+  void alloc_bucket_set(BucketSet& b);
+  void free_bucket_set(BucketSet& b);
+  void clear_bucket_set(BucketSet& b);
+  size_t bucket_set_bytes() const;
   void ensure_tiles(const Shape& s);
   void enqueue(const Shape& s, int n_steps, const float* noise, size_t stride);
   Shape resolve_for(Shape s) const;

@@ -3,10 +3,11 @@
 // Forward: the tiled step kernel, with the state copied to a checkpoint every ckpt_every steps.
 // Reverse, segment by segment from the last: the segment is recomputed from its checkpoint by
 // the recording variant of the same kernel (bitwise the forward's states), which stores every
-// pre-step state, the force and alignment sums and the seam-wrap turns; then, step by step
-// backwards, the step's vertex buckets are rebuilt from its recorded vertex ids and the two
-// reverse kernels (per-particle factors, then the transposed neighbour gather) map the adjoint
-// of the new state to the adjoint of the old one.
+// pre-step state, the force and alignment sums and the seam-wrap turns, and keeps the vertex
+// buckets of every state of the segment (when they fit in about 16 GB; otherwise each reverse
+// step rebuilds its state's buckets from the recorded vertex ids); then, step by step backwards,
+// the two reverse kernels (per-particle factors, then the transposed neighbour gather over the
+// step's buckets) map the adjoint of the new state to the adjoint of the old one.
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -53,6 +54,7 @@ void Engine::adjoint_release() {
   dfree(a.gF); dfree(a.gSC); dfree(a.part); dfree(a.acc); dfree(a.out_seed); dfree(a.out_red);
   dfree(a.vx); dfree(a.vy); dfree(a.r3x); dfree(a.r3y); dfree(a.r3z); dfree(a.face); dfree(a.nbr);
   dfree(a.rows); dfree(a.first_bad); dfree(a.partial); dfree(a.sexp);
+  for (BucketSet& b : a.sbs) free_bucket_set(b);
   a = AdjointBuffers{};
 }
 
@@ -85,6 +87,15 @@ void Engine::adjoint_buffers(int K, int C) {
     a.tp_fs = dalloc<float4>((size_t)K * N);
     a.tp_q = dalloc<int>((size_t)K * N);
     a.K = K;
+  }
+  // the buckets of every state of a segment, kept for its reverse steps, within about 16 GB
+  // (otherwise each reverse step re-buckets its state)
+  const size_t want = (size_t)K + 2;
+  if (a.sbs.size() < want && want * bucket_set_bytes() <= (size_t(16) << 30)) {
+    while (a.sbs.size() < want) {
+      a.sbs.emplace_back();
+      alloc_bucket_set(a.sbs.back());
+    }
   }
   if (a.partial_n < n_logical) {
     dfree(a.partial);
@@ -127,20 +138,19 @@ int32_t Engine::adjoint(int T, const float* nz, size_t stride, int K, const tiss
   HIP_OK(hipMemsetAsync(a.acc, 0, kAdjAcc * N * sizeof(double), stream));
   HIP_OK(hipMemsetAsync(a.sexp, 0, R * sizeof(int), stream));
   int cur = 0;
-  const size_t cells = (size_t)R * md.nv;
   for (int c = C - 1; c >= 0; --c) {
     const int t0 = c * K, kc = std::min(K, T - t0);
     // recompute the segment from its checkpoint, recording the tape
     HIP_OK(hipMemcpyAsync(a.tp_xycs, a.ck_xycs + c * N, N * sizeof(float4), hipMemcpyDeviceToDevice, stream));
     HIP_OK(hipMemcpyAsync(a.tp_th, a.ck_th + c * N, N * sizeof(float), hipMemcpyDeviceToDevice, stream));
     HIP_OK(hipMemcpyAsync(a.tp_vid, a.ck_vid + c * N, N * sizeof(int), hipMemcpyDeviceToDevice, stream));
-    for (int k = 1; k < 3; ++k) {
-      HIP_OK(hipMemsetAsync(bs[k].count, 0, cells * sizeof(int), stream));
-      HIP_OK(hipMemsetAsync(bs[k].ovf_n, 0, R * sizeof(int), stream));
-      HIP_OK(hipMemsetAsync(bs[k].multi_n, 0, R * sizeof(int), stream));
-      HIP_OK(hipMemsetAsync(bs[k].big_n, 0, R * sizeof(int), stream));
-    }
-    bucket_state(a.tp_vid, 0);
+    // bucket sets of the segment: the states' own (set q holds state q's buckets) when they fit,
+    // else the three rotating sets, with each reverse step re-bucketing its state
+    const bool keep = a.sbs.size() >= (size_t)kc + 2;
+    auto set = [&](int q) -> BucketSet& { return keep ? a.sbs[q] : bs[q % 3]; };
+    clear_bucket_set(set(1));
+    clear_bucket_set(set(2));
+    bucket_state_into(a.tp_vid, set(0));
     HIP_OK(launch_flags_begin(R, d_halt, a.rows, a.first_bad, stream));
     for (int q = 0; q < kc; ++q) {
       TiledIO io{};
@@ -156,17 +166,17 @@ int32_t Engine::adjoint(int T, const float* nz, size_t stride, int K, const tiss
       io.step = q;
       io.tape_fs = a.tp_fs + q * N;
       io.tape_q = a.tp_q + q * N;
-      HIP_OK(launch_step_tiled(s.lanes, s.block, true, md, d_reps, d_tiles, n_phys, bs[q % 3], bs[(q + 1) % 3],
-                               bs[(q + 2) % 3], io, stream));
-      HIP_OK(launch_bucket_finalize(s.block, d_reps, d_tiles, n_phys, md.nv, bs[(q + 1) % 3], stream));
+      HIP_OK(launch_step_tiled(s.lanes, s.block, true, md, d_reps, d_tiles, n_phys, set(q), set(q + 1), set(q + 2), io,
+                               stream));
+      HIP_OK(launch_bucket_finalize(s.block, d_reps, d_tiles, n_phys, md.nv, set(q + 1), stream));
     }
     // backwards through the segment
     for (int q = kc - 1; q >= 0; --q) {
-      bucket_state(a.tp_vid + q * N, 0);
+      if (!keep) bucket_state_into(a.tp_vid + q * N, bs[0]);
       HIP_OK(launch_adj_local(d_reps, d_rid, ntot, L, d_halt, a.sexp, a.tp_xycs + q * N, a.tp_th + q * N, a.tp_fs + q * N,
                               a.tp_q + q * N, a.lr[cur], a.lt[cur], a.gF, a.gSC, a.part, a.acc, stream));
-      HIP_OK(launch_adj_gather(s.lanes, md, d_reps, d_rid, ntot, bs[0], d_halt, a.tp_xycs + q * N, a.tp_vid + q * N,
-                               a.gF, a.gSC, a.part, a.lr[1 - cur], a.lt[1 - cur], stream));
+      HIP_OK(launch_adj_gather(s.lanes, md, d_reps, d_rid, ntot, keep ? a.sbs[q] : bs[0], d_halt, a.tp_xycs + q * N,
+                               a.tp_vid + q * N, a.gF, a.gSC, a.part, a.lr[1 - cur], a.lt[1 - cur], stream));
       cur = 1 - cur;
       HIP_OK(launch_adj_rescale(R, d_reps, a.lr[cur], a.lt[cur], a.sexp, stream));
     }
