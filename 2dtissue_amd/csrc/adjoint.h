@@ -24,18 +24,20 @@ struct LossDev {
 hipError_t launch_adj_seed(int R, const ReplicaDesc* reps, const float4* xycs, const LossDev& L, const unsigned* halt,
                            float2* lr, float* lt, double* out, hipStream_t st);
 // per particle factors of step t's reverse (see adjoint.hip), accumulators acc[kAdjAcc][ntot];
-// the adjoint state of replica r is stored scaled by 2^-sexp[r]
+// the adjoint state of replica r is stored scaled by 2^-E_r: the kernel reads E_r from sexp_in and
+// the max |lambda| of its input from mx_in (the previous gather's), scales its input by a power of
+// two when that max exceeds 2^21, writes the new E_r to sexp_out and clears mx_next
 hipError_t launch_adj_local(const ReplicaDesc* reps, const int* rid, int ntot, const LossDev& L, const unsigned* halt,
-                            const int* sexp, const float4* xycs, const float* th, const float4* fs, const int* quarter,
+                            const int* sexp_in, int* sexp_out, const unsigned* mx_in, unsigned* mx_next,
+                            const float4* xycs, const float* th, const float4* fs, const int* quarter,
                             const float2* lr_in, const float* lt_in, float2* gF, float2* gSC, float4* part,
                             double* acc, hipStream_t st);
-// the transposed neighbour sums over step t's vertex buckets bs
+// the transposed neighbour sums over step t's vertex buckets bs; mx_out[r] = max |lambda| of the
+// output of replica r (float bits, combined with atomicMax: clear it before)
 hipError_t launch_adj_gather(int lanes, const tdev::MeshDev& mesh, const ReplicaDesc* reps, const int* rid, int ntot,
                              const BucketSet& bs, const unsigned* halt, const float4* xycs, const int* vid,
                              const float2* gF, const float2* gSC, const float4* part, float2* lr_out, float* lt_out,
-                             hipStream_t st);
-// exact power-of-two rescaling of replica r's adjoint state when |lambda| > 2^21; sexp[r] += shift
-hipError_t launch_adj_rescale(int R, const ReplicaDesc* reps, float2* lr, float* lt, int* sexp, hipStream_t st);
+                             unsigned* mx_out, hipStream_t st);
 // out[kAdjAcc r + k] = sum over replica r of acc[k]
 hipError_t launch_adj_reduce(int R, const ReplicaDesc* reps, int ntot, const double* acc, double* out, hipStream_t st);
 

@@ -17,7 +17,10 @@
 // factors, k_adj_gather walks the same geodesic rows and vertex buckets as the forward step.
 // The adjoint state of replica r is held as lambda * 2^-E_r (FP32, E_r >= 0 in sexp[r]): the map
 // is linear in lambda, so only the loss sources are scaled by 2^-E_r and the FP64 parameter
-// accumulators by 2^E_r; k_adj_rescale raises E_r whenever |lambda| outgrows 2^20.
+// accumulators by 2^E_r. E_r is raised whenever |lambda| outgrows 2^20: k_adj_gather records each
+// replica's max |lambda| (one atomic per wave), and the next k_adj_local scales its inputs by the
+// matching power of two and carries E_r forward (E and the max alternate between two slots per
+// replica by step parity, so no kernel reads a slot another thread of it writes).
 #include <hip/hip_runtime.h>
 
 #include "adjoint.h"
@@ -96,13 +99,31 @@ __global__ void k_adj_seed(const ReplicaDesc* reps, const float4* xycs, LossDev 
   }
 }
 
+// power-of-two exponent that brings a replica's max |lambda| (float bits of a non-negative value)
+// below 2^21; 0 when no rescale is needed or the max is not finite
+__device__ __forceinline__ int rescale_exponent(unsigned mx_bits) {
+  const float mx = __uint_as_float(mx_bits);
+  if (!(mx > 0.f) || !isfinite(mx)) return 0;
+  const int s = ilogbf(mx) - 20;
+  return s > 0 ? s : 0;
+}
+
 __global__ void k_adj_local(const ReplicaDesc* reps, const int* rid, int ntot, LossDev L, const unsigned* halt,
-                            const int* sexp, const float4* xycs, const float* th, const float4* fs, const int* quarter,
+                            const int* sexp_in, int* sexp_out, const unsigned* mx_in, unsigned* mx_next,
+                            const float4* xycs, const float* th, const float4* fs, const int* quarter,
                             const float2* lr_in, const float* lt_in, float2* gF, float2* gSC, float4* part,
                             double* acc) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ntot) return;
   const int r = rid[g];
+  // rescale of the incoming adjoint (exact: a power of two); the replica's first particle carries
+  // the exponent forward and clears the max slot the following gather fills
+  const int s_up = rescale_exponent(mx_in[r]);
+  const int E = sexp_in[r] + s_up;
+  if (g == reps[r].off) {
+    sexp_out[r] = E;
+    mx_next[r] = 0u;
+  }
   if (halt[r]) {
     gF[g] = make_float2(0.f, 0.f);
     gSC[g] = make_float2(0.f, 0.f);
@@ -110,14 +131,15 @@ __global__ void k_adj_local(const ReplicaDesc* reps, const int* rid, int ntot, L
     return;
   }
   const StepParams& prm = reps[r].prm;
-  const int E = sexp[r];
+  const float down = ldexpf(1.f, -s_up);
   const float src = ldexpf(1.f, -E);        // loss sources in the scaled units
   const double unscale = ldexp(1.0, E);     // back to true units for the accumulators
   const float4 me = xycs[g];
   const float theta = th[g];
   const float4 f = fs[g];  // (Fx, Fy, sum cos, sum sin)
-  const float2 lp = rot_t(quarter[g], lr_in[g]);
-  const float ltn = lt_in[g];
+  const float2 lr_s = lr_in[g];
+  const float2 lp = rot_t(quarter[g], make_float2(lr_s.x * down, lr_s.y * down));
+  const float ltn = lt_in[g] * down;
   const float m = sqrtf(f.x * f.x + f.y * f.y);
   const float s = prm.v0 + prm.mu * m;
   const float rdx = me.z * s, rdy = me.w * s;
@@ -156,76 +178,68 @@ template <int LANES>
 __global__ void __launch_bounds__(256) k_adj_gather(MeshDev mesh, const ReplicaDesc* reps, const int* rid, int ntot,
                                                    BucketSet bs, const unsigned* halt, const float4* xycs,
                                                    const int* vid, const float2* gF, const float2* gSC,
-                                                   const float4* part, float2* lr_out, float* lt_out) {
+                                                   const float4* part, float2* lr_out, float* lt_out,
+                                                   unsigned* mx_out) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int g = tid / LANES, lane = tid & (LANES - 1);
-  if (g >= ntot) return;  // whole groups (ntot * LANES threads are grouped by LANES)
-  const int r = rid[g];
-  if (halt[r]) {
+  const int r = g < ntot ? rid[g] : -1;  // -1: past the end (the lanes stay for the wave reduction)
+  float mx = 0.f;                        // |lambda| of this particle (on lane 0 of its group)
+  if (r >= 0 && halt[r]) {
     if (lane == 0) { lr_out[g] = make_float2(0.f, 0.f); lt_out[g] = 0.f; }
-    return;
+  } else if (r >= 0) {
+    const ReplicaDesc& rep = reps[r];
+    const StepParams& prm = rep.prm;
+    const int nv = mesh.nv;
+    const BucketsIn b{bs.count + (size_t)r * nv, bs.slot + (size_t)r * nv * kSlots, bs.big + rep.off};
+    const float2 mine = gF[g];
+    float ax = 0.f, ay = 0.f, aS = 0.f, aC = 0.f;
+    const int v = vid[g];
+    const int rs = mesh.row_ptr[v], re = mesh.row_ptr[v + 1];
+    for (int e = rs + lane; e < re; e += LANES) {
+      const int2 ent = mesh.nbr[e];
+      const float d = __int_as_float(ent.y);
+      if (d > prm.t_near) break;  // rows ascend in distance
+      const float c = pair_coef(d, prm);
+      for_bucket(b, ent.x, [&](int j) {
+        const float2 oj = gF[j];
+        const float2 sc = gSC[j];
+        if (j != g) {
+          ax += c * (mine.x - oj.x);
+          ay += c * (mine.y - oj.y);
+        }
+        aS += sc.x;
+        aC += sc.y;
+      });
+    }
+    ax = group_sum<LANES>(ax);
+    ay = group_sum<LANES>(ay);
+    aS = group_sum<LANES>(aS);
+    aC = group_sum<LANES>(aC);
+    if (lane == 0) {
+      const float4 pt = part[g];
+      const float4 me = xycs[g];
+      const float2 o = make_float2(pt.x + ax, pt.y + ay);
+      const float ot = pt.z + me.z * aS - me.w * aC;
+      lr_out[g] = o;
+      lt_out[g] = ot;
+      mx = fmaxf(fabsf(o.x), fmaxf(fabsf(o.y), fabsf(ot)));  // fmaxf skips NaN
+    }
   }
-  const ReplicaDesc& rep = reps[r];
-  const StepParams& prm = rep.prm;
-  const int nv = mesh.nv;
-  const BucketsIn b{bs.count + (size_t)r * nv, bs.slot + (size_t)r * nv * kSlots, bs.big + rep.off};
-  const float2 mine = gF[g];
-  float ax = 0.f, ay = 0.f, aS = 0.f, aC = 0.f;
-  const int v = vid[g];
-  const int rs = mesh.row_ptr[v], re = mesh.row_ptr[v + 1];
-  for (int e = rs + lane; e < re; e += LANES) {
-    const int2 ent = mesh.nbr[e];
-    const float d = __int_as_float(ent.y);
-    if (d > prm.t_near) break;  // rows ascend in distance
-    const float c = pair_coef(d, prm);
-    for_bucket(b, ent.x, [&](int j) {
-      const float2 oj = gF[j];
-      const float2 sc = gSC[j];
-      if (j != g) {
-        ax += c * (mine.x - oj.x);
-        ay += c * (mine.y - oj.y);
-      }
-      aS += sc.x;
-      aC += sc.y;
-    });
-  }
-  ax = group_sum<LANES>(ax);
-  ay = group_sum<LANES>(ay);
-  aS = group_sum<LANES>(aS);
-  aC = group_sum<LANES>(aC);
-  if (lane == 0) {
-    const float4 pt = part[g];
-    const float4 me = xycs[g];
-    lr_out[g] = make_float2(pt.x + ax, pt.y + ay);
-    lt_out[g] = pt.z + me.z * aS - me.w * aC;
-  }
-}
-
-// one block per replica: keep |lambda| below 2^21 by exact power-of-two rescaling
-__global__ void k_adj_rescale(const ReplicaDesc* reps, float2* lr, float* lt, int* sexp) {
-  __shared__ float tmp[kAdjBlock / 64];
-  const int r = blockIdx.x;
-  const int off = reps[r].off, n = reps[r].n;
-  float mx = 0.f;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const float2 a = lr[off + i];
-    mx = fmaxf(mx, fmaxf(fabsf(a.x), fmaxf(fabsf(a.y), fabsf(lt[off + i]))));
-  }
+  // each replica's max |lambda| over the wave: the wave's particles of one replica occupy a
+  // contiguous range of lanes, so an inclusive max-scan restricted to that range leaves the max
+  // on its last lane, which posts it (float bits of non-negative values order like the values)
+  // (replicas are concatenated, so r never decreases along the wave)
+  const int me_lane = (int)__lane_id();
+  const int r_prev = __shfl_up(r, 1, 64), r_next = __shfl_down(r, 1, 64);
+  const unsigned long long heads = __ballot(me_lane == 0 || r_prev != r);
+  const unsigned long long upto = me_lane == 63 ? ~0ull : (2ull << me_lane) - 1ull;
+  const int first = 63 - __clzll(heads & upto);
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
-  if ((threadIdx.x & 63) == 0) tmp[threadIdx.x >> 6] = mx;
-  __syncthreads();
-  for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) mx = fmaxf(mx, tmp[w]);
-  if (!(mx > 0.f) || !isfinite(mx)) return;  // uniform over the block
-  const int s = ilogbf(mx) - 20;
-  if (s <= 0) return;
-  const float f = ldexpf(1.f, -s);
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const float2 a = lr[off + i];
-    lr[off + i] = make_float2(a.x * f, a.y * f);
-    lt[off + i] *= f;
+  for (int o = 1; o < 64; o <<= 1) {
+    const float up = __shfl_up(mx, o, 64);
+    if (me_lane - o >= first) mx = fmaxf(mx, up);
   }
-  if (threadIdx.x == 0) sexp[r] += s;
+  if ((me_lane == 63 || r_next != r) && r >= 0 && mx > 0.f) atomicMax(&mx_out[r], __float_as_uint(mx));
 }
 
 // one block per replica: sums of the per-particle accumulators
@@ -252,27 +266,23 @@ hipError_t launch_adj_seed(int R, const ReplicaDesc* reps, const float4* xycs, c
 }
 
 hipError_t launch_adj_local(const ReplicaDesc* reps, const int* rid, int ntot, const LossDev& L, const unsigned* halt,
-                            const int* sexp, const float4* xycs, const float* th, const float4* fs, const int* quarter,
+                            const int* sexp_in, int* sexp_out, const unsigned* mx_in, unsigned* mx_next,
+                            const float4* xycs, const float* th, const float4* fs, const int* quarter,
                             const float2* lr_in, const float* lt_in, float2* gF, float2* gSC, float4* part,
                             double* acc, hipStream_t st) {
-  hipLaunchKernelGGL(k_adj_local, dim3(cdiv(ntot, 256)), dim3(256), 0, st, reps, rid, ntot, L, halt, sexp, xycs, th,
-                     fs, quarter, lr_in, lt_in, gF, gSC, part, acc);
-  return hipGetLastError();
-}
-
-hipError_t launch_adj_rescale(int R, const ReplicaDesc* reps, float2* lr, float* lt, int* sexp, hipStream_t st) {
-  hipLaunchKernelGGL(k_adj_rescale, dim3(R), dim3(kAdjBlock), 0, st, reps, lr, lt, sexp);
+  hipLaunchKernelGGL(k_adj_local, dim3(cdiv(ntot, 256)), dim3(256), 0, st, reps, rid, ntot, L, halt, sexp_in, sexp_out,
+                     mx_in, mx_next, xycs, th, fs, quarter, lr_in, lt_in, gF, gSC, part, acc);
   return hipGetLastError();
 }
 
 hipError_t launch_adj_gather(int lanes, const MeshDev& mesh, const ReplicaDesc* reps, const int* rid, int ntot,
                              const BucketSet& bs, const unsigned* halt, const float4* xycs, const int* vid,
                              const float2* gF, const float2* gSC, const float4* part, float2* lr_out, float* lt_out,
-                             hipStream_t st) {
+                             unsigned* mx_out, hipStream_t st) {
   const int blocks = cdiv((long)ntot * lanes, 256);
 #define TISSUE_ADJ_GATHER(L)                                                                                     \
   hipLaunchKernelGGL(k_adj_gather<L>, dim3(blocks), dim3(256), 0, st, mesh, reps, rid, ntot, bs, halt, xycs, vid, \
-                     gF, gSC, part, lr_out, lt_out)
+                     gF, gSC, part, lr_out, lt_out, mx_out)
   switch (lanes) {
     case 1: TISSUE_ADJ_GATHER(1); break;
     case 2: TISSUE_ADJ_GATHER(2); break;

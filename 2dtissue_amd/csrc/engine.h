@@ -137,7 +137,8 @@ struct Engine {
     float *vx = nullptr, *vy = nullptr, *r3x = nullptr, *r3y = nullptr, *r3z = nullptr;
     int *face = nullptr, *nbr = nullptr;
     unsigned* rows = nullptr; int* first_bad = nullptr;
-    int* sexp = nullptr;  // per replica: the adjoint state is stored scaled by 2^-sexp
+    int* sexp = nullptr;  // [2][R]: the adjoint state is stored scaled by 2^-sexp (slots alternate by step)
+    unsigned* mx = nullptr;  // [2][R]: max |lambda| of the last gather's output (float bits)
     OrderPartial* partial = nullptr; int partial_n = 0;
     // bucket sets of every state of a segment (kept for the reverse steps), when they fit
     std::vector<BucketSet> sbs;

@@ -53,7 +53,7 @@ void Engine::adjoint_release() {
   for (int k = 0; k < 2; ++k) { dfree(a.lr[k]); dfree(a.lt[k]); }
   dfree(a.gF); dfree(a.gSC); dfree(a.part); dfree(a.acc); dfree(a.out_seed); dfree(a.out_red);
   dfree(a.vx); dfree(a.vy); dfree(a.r3x); dfree(a.r3y); dfree(a.r3z); dfree(a.face); dfree(a.nbr);
-  dfree(a.rows); dfree(a.first_bad); dfree(a.partial); dfree(a.sexp);
+  dfree(a.rows); dfree(a.first_bad); dfree(a.partial); dfree(a.sexp); dfree(a.mx);
   for (BucketSet& b : a.sbs) free_bucket_set(b);
   a = AdjointBuffers{};
 }
@@ -72,7 +72,8 @@ void Engine::adjoint_buffers(int K, int C) {
     a.face = dalloc<int>(N); a.nbr = dalloc<int>(N);
     a.rows = dalloc<unsigned>(3 * (size_t)R);
     a.first_bad = dalloc<int>(R);
-    a.sexp = dalloc<int>(R);
+    a.sexp = dalloc<int>(2 * (size_t)R);
+    a.mx = dalloc<unsigned>(2 * (size_t)R);
   }
   if (C > a.C) {
     dfree(a.ck_xycs); dfree(a.ck_th); dfree(a.ck_vid);
@@ -136,7 +137,9 @@ int32_t Engine::adjoint(int T, const float* nz, size_t stride, int K, const tiss
   HIP_OK(hipEventRecord(ev0, stream));
   HIP_OK(launch_adj_seed(R, d_reps, xycs[sphase], L, d_halt, a.lr[0], a.lt[0], a.out_seed, stream));
   HIP_OK(hipMemsetAsync(a.acc, 0, kAdjAcc * N * sizeof(double), stream));
-  HIP_OK(hipMemsetAsync(a.sexp, 0, R * sizeof(int), stream));
+  HIP_OK(hipMemsetAsync(a.sexp, 0, 2 * R * sizeof(int), stream));
+  HIP_OK(hipMemsetAsync(a.mx, 0, 2 * R * sizeof(unsigned), stream));
+  // lr[cur], lt[cur]: the adjoint of the current state; slot cur of sexp and mx: its exponent and max
   int cur = 0;
   for (int c = C - 1; c >= 0; --c) {
     const int t0 = c * K, kc = std::min(K, T - t0);
@@ -173,12 +176,13 @@ int32_t Engine::adjoint(int T, const float* nz, size_t stride, int K, const tiss
     // backwards through the segment
     for (int q = kc - 1; q >= 0; --q) {
       if (!keep) bucket_state_into(a.tp_vid + q * N, bs[0]);
-      HIP_OK(launch_adj_local(d_reps, d_rid, ntot, L, d_halt, a.sexp, a.tp_xycs + q * N, a.tp_th + q * N, a.tp_fs + q * N,
+      HIP_OK(launch_adj_local(d_reps, d_rid, ntot, L, d_halt, a.sexp + cur * R, a.sexp + (1 - cur) * R, a.mx + cur * R,
+                              a.mx + (1 - cur) * R, a.tp_xycs + q * N, a.tp_th + q * N, a.tp_fs + q * N,
                               a.tp_q + q * N, a.lr[cur], a.lt[cur], a.gF, a.gSC, a.part, a.acc, stream));
       HIP_OK(launch_adj_gather(s.lanes, md, d_reps, d_rid, ntot, keep ? a.sbs[q] : bs[0], d_halt, a.tp_xycs + q * N,
-                               a.tp_vid + q * N, a.gF, a.gSC, a.part, a.lr[1 - cur], a.lt[1 - cur], stream));
+                               a.tp_vid + q * N, a.gF, a.gSC, a.part, a.lr[1 - cur], a.lt[1 - cur],
+                               a.mx + (1 - cur) * R, stream));
       cur = 1 - cur;
-      HIP_OK(launch_adj_rescale(R, d_reps, a.lr[cur], a.lt[cur], a.sexp, stream));
     }
   }
   HIP_OK(launch_adj_reduce(R, d_reps, ntot, a.acc, a.out_red, stream));
@@ -191,7 +195,7 @@ int32_t Engine::adjoint(int T, const float* nz, size_t stride, int K, const tiss
   HIP_OK(hipMemcpyAsync(red.data(), a.out_red, red.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
   HIP_OK(hipMemcpyAsync(lr.data(), a.lr[cur], N * sizeof(float2), hipMemcpyDeviceToHost, stream));
   HIP_OK(hipMemcpyAsync(lt.data(), a.lt[cur], N * sizeof(float), hipMemcpyDeviceToHost, stream));
-  HIP_OK(hipMemcpyAsync(sexp.data(), a.sexp, R * sizeof(int), hipMemcpyDeviceToHost, stream));
+  HIP_OK(hipMemcpyAsync(sexp.data(), a.sexp + cur * R, R * sizeof(int), hipMemcpyDeviceToHost, stream));
   HIP_OK(hipStreamSynchronize(stream));
   float rev = 0.f;
   HIP_OK(hipEventElapsedTime(&rev, ev0, ev1));
