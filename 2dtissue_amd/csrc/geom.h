@@ -192,11 +192,88 @@ __host__ __device__ inline bool seam_wrap(double ax, double ay, double& px, doub
   return ok;
 }
 
+// The border tests and the crossing rewritten for the lane-parallel loop: the same roundings as
+// border_test and cross_border, with the choices between borders made by exact negations and
+// sign-bit flips instead of selects, so one crossing is about 100 instructions instead of 140.
+// The identities used (round to nearest is symmetric): fl(-0.0 - x) = -x bit for bit (zero
+// signs included), fl(1 - x) = omx, fl(2 - x) as the reference forms it.
+//
+// Border k's test in the form one lane evaluates with its lane constants: lr = (k < 2),
+// ca = (k == 1 ? 1 : -0), cb = (k == 2 ? 1 : -0). Then a = fl(ca - ax) and b = fl(cb - ay) are
+// border_test's (k == 1 ? omx : -ax) and (k == 2 ? omy : -ay); its numerator is a (left /
+// right) or -b (up / down), its determinant dx or -dy, and "the entry point lies on this edge"
+// is a == 0 (left / right) or b == 0 (up / down), with the point's other coordinate in [0, 1].
+// Folding the determinant's sign into both numerators leaves the numerator's flip independent
+// of lr: nf = flip(lr ? a : b, sign(lr ? dx : dy)).
+__host__ __device__ inline bool border_pass(bool lr, double ca, double cb, double ax, double ay, double dx,
+                                            double dy) {
+  const double a = ca - ax, b = cb - ay;
+  const double N = lr ? a : b, D = lr ? dx : dy, W = lr ? ay : ax;
+  const double sv = a * dy - b * dx;
+  const uint64_t f = bits64(D) & kSignBit;
+  const double ad = fabs(D);
+  const double nf = flip(N, f), svf = flip(sv, f ^ (lr ? 0ull : kSignBit));
+  const int off_edge = (int)!(N == 0.0) | (int)!(W >= 0.0) | (int)!(W <= 1.0);
+  return (off_edge & (int)!(ad < 1e-9) & folded_in_unit(nf, ad) & folded_in_unit(svf, ad)) != 0;
+}
+
+// cross_border for border b (4: none passed) with the per-border choices written as sign flips:
+// numerator flip(fl(c1 - X), sgn), X = ax (left / right) or ay, c1 = 1 for right / up else -0;
+// determinant flip(lr ? dx : dy, sgn); new end point flip(fl(cx - oy), sgn ^ sign),
+// flip(fl(cy - ox), sgn) with cx = 2 for up, cy = 2 for right, else -0; sgn = 0 for left / right,
+// the sign bit otherwise.
+__host__ __device__ inline void cross_border_flip(int b, double& ax, double& ay, double& px, double& py, double dx,
+                                                  double dy, double& dtheta, int& quarter) {
+  const bool side = b < 2;
+  const uint64_t sgn = side ? 0ull : kSignBit;
+  const double X = side ? ax : ay;
+  const double c1 = ((b == 1) | (b == 2)) ? 1.0 : -0.0;
+  const double nt = flip(c1 - X, sgn);
+  const double det = flip(side ? dx : dy, sgn);
+  const double t = quotient(nt, det, quotient_reciprocal(det));
+  double ex = ax + t * dx, ey = ay + t * dy;
+  if (fabs(ex) < 1e-3) ex = 0.0;
+  if (fabs(ey) < 1e-3) ey = 0.0;
+  if (b == 4) {
+    ex = ax;
+    ey = ay;
+  }
+  const double cx = b == 2 ? 2.0 : -0.0, cy = b == 1 ? 2.0 : -0.0;
+  const double ox = px, oy = py;
+  px = flip(cx - oy, sgn ^ kSignBit);
+  py = flip(cy - ox, sgn);
+  dtheta -= side ? 0.5 * kPi : 1.5 * kPi;
+  quarter += side ? -1 : 1;
+  ax = ey;
+  ay = ex;
+}
+
+// One lane-group's choice on the host: border_pass for the four borders in order, the first
+// passing one crossed (tests/hostcheck compares this with seam_wrap bit for bit)
+__host__ __device__ inline bool seam_wrap_flip(double ax, double ay, double& px, double& py, double& dtheta,
+                                               int max_crossings, int* crossings_out = nullptr,
+                                               int* quarter_out = nullptr) {
+  int crossings = 0, quarter = 0;
+  dtheta = 0.0;
+  bool ok = true;
+  while (!inside_unit(px, py)) {
+    if (crossings >= max_crossings) { ok = false; break; }
+    ++crossings;
+    const double dx = px - ax, dy = py - ay;
+    int b = 4;
+    for (int k = 3; k >= 0; --k)
+      if (border_pass(k < 2, k == 1 ? 1.0 : -0.0, k == 2 ? 1.0 : -0.0, ax, ay, dx, dy)) b = k;
+    cross_border_flip(b, ax, ay, px, py, dx, dy, dtheta, quarter);
+  }
+  if (crossings_out) *crossings_out = crossings;
+  if (quarter_out) *quarter_out = quarter & 3;
+  return ok;
+}
+
 // The same loop run by the LANES lanes of one particle (a lane group aligned within the wave,
 // all of its lanes active): the four border tests are spread over the lanes (lane % 4 tests
-// border lane % 4; with two lanes, each lane tests two borders) and the first passing border is
-// read from a wave ballot, so each lane evaluates one or two tests instead of four. Same tests,
-// same arithmetic, same decisions as seam_wrap.
+// border lane % 4; with two lanes, lane k tests border k and border k + 2) and the first passing
+// border is read from a wave ballot. Same decisions and the same FP64 values as seam_wrap.
 template <int LANES>
 __device__ inline bool seam_wrap_group(double ax, double ay, double& px, double& py, double& dtheta, int max_crossings,
                                        int lane, int* quarter_out) {
@@ -204,8 +281,12 @@ __device__ inline bool seam_wrap_group(double ax, double ay, double& px, double&
     (void)lane;
     return seam_wrap(ax, ay, px, py, dtheta, max_crossings, nullptr, quarter_out);
   } else {
+    const unsigned base = __lane_id() & ~(unsigned)(LANES - 1);
+    // lane constants of the lane's border(s): border k (four lanes or more), or borders k and
+    // k + 2 (two lanes), with ca, cb as border_pass takes them
     const int k = LANES >= 4 ? (lane & 3) : (lane & 1);
-    const int base = (int)(__lane_id() & ~(unsigned)(LANES - 1));
+    const double ca = k == 1 ? 1.0 : -0.0;
+    const double cb = (LANES >= 4 ? k == 2 : k == 0) ? 1.0 : -0.0;
     int crossings = 0;
     int quarter = 0;
     dtheta = 0.0;
@@ -213,18 +294,19 @@ __device__ inline bool seam_wrap_group(double ax, double ay, double& px, double&
     while (!inside_unit(px, py)) {
       if (crossings >= max_crossings) { ok = false; break; }
       ++crossings;
-      const double dx = px - ax, dy = py - ay, omx = 1.0 - ax, omy = 1.0 - ay;
-      const double rx = quotient_reciprocal(dx), ry = quotient_reciprocal(-dy);
+      const double dx = px - ax, dy = py - ay;
       unsigned bits;
       if constexpr (LANES >= 4) {
-        bits = (unsigned)(__ballot(border_test(k, ax, ay, dx, dy, omx, omy)) >> base) & 0xFu;
+        bits = (unsigned)(__builtin_amdgcn_ballot_w64(border_pass(k < 2, ca, cb, ax, ay, dx, dy)) >> base);
       } else {
-        const unsigned lo = (unsigned)(__ballot(border_test(k, ax, ay, dx, dy, omx, omy)) >> base) & 0x3u;
-        const unsigned hi = (unsigned)(__ballot(border_test(k + 2, ax, ay, dx, dy, omx, omy)) >> base) & 0x3u;
+        const unsigned lo = (unsigned)(__builtin_amdgcn_ballot_w64(border_pass(true, ca, -0.0, ax, ay, dx, dy)) >> base) & 0x3u;
+        const unsigned hi = (unsigned)(__builtin_amdgcn_ballot_w64(border_pass(false, -0.0, cb, ax, ay, dx, dy)) >> base) & 0x3u;
         bits = lo | (hi << 2);
       }
-      const int border = bits ? __builtin_ctz(bits) : 4;
-      cross_border(border, ax, ay, px, py, dx, dy, omx, omy, rx, ry, dtheta, quarter);
+      // first passing border of the group (bits above the group's four belong to the next group:
+      // a lowest set bit at 4 or above, or none, means none of ours passed)
+      const unsigned low = (unsigned)__builtin_ctz(bits | 0x10u);
+      cross_border_flip((int)low, ax, ay, px, py, dx, dy, dtheta, quarter);
     }
     if (quarter_out) *quarter_out = quarter & 3;
     return ok;

@@ -31,6 +31,30 @@ void hc_seam_wrap_many(int n, const double* a, const double* p, int max_crossing
   }
 }
 
+// the lane-parallel loop's formulation (border_pass, cross_border_flip) run on the host
+void hc_seam_wrap_flip_many(int n, const double* a, const double* p, int max_crossings, double* out_p, double* dth,
+                            int* ok, int* crossings, int* quarter) {
+  for (int i = 0; i < n; ++i) {
+    double px = p[2 * i], py = p[2 * i + 1], dt = 0.0;
+    int c = 0, q = 0;
+    ok[i] = tdev::seam_wrap_flip(a[2 * i], a[2 * i + 1], px, py, dt, max_crossings, &c, &q) ? 1 : 0;
+    out_p[2 * i] = px;
+    out_p[2 * i + 1] = py;
+    dth[i] = dt;
+    crossings[i] = c;
+    quarter[i] = q;
+  }
+}
+
+void hc_seam_wrap_quarter_many(int n, const double* a, const double* p, int max_crossings, int* quarter) {
+  for (int i = 0; i < n; ++i) {
+    double px = p[2 * i], py = p[2 * i + 1], dt = 0.0;
+    int q = 0;
+    tdev::seam_wrap(a[2 * i], a[2 * i + 1], px, py, dt, max_crossings, nullptr, &q);
+    quarter[i] = q;
+  }
+}
+
 // FP32 branch-free squared distance and the FP64 region-by-region distance, for n (point,
 // triangle) pairs: tri = (ax, ay, bx, by, cx, cy) per pair
 void hc_tri_dist_many(int n, const float* pts, const float* tri, float* d2_f32, double* d_f64) {

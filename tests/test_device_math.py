@@ -90,6 +90,62 @@ def test_seam_wrap_matches_oracle_adversarial(H, O):
     assert np.array_equal(dth[both_ok], h[both_ok])
 
 
+def wrap_flip_vs_direct(H, a, b, maxc):
+    n = len(a)
+    a = np.ascontiguousarray(a, np.float64); b = np.ascontiguousarray(b, np.float64)
+    res = []
+    for fn in ("direct", "flip"):
+        out = np.zeros((n, 2)); dth = np.zeros(n); ok = np.zeros(n, np.int32); cr = np.zeros(n, np.int32)
+        q = np.zeros(n, np.int32)
+        if fn == "direct":
+            H.hc_seam_wrap_many(n, p(a, C.c_double), p(b, C.c_double), maxc, p(out, C.c_double), p(dth, C.c_double),
+                                p(ok, C.c_int32), p(cr, C.c_int32))
+            H.hc_seam_wrap_quarter_many(n, p(a, C.c_double), p(b, C.c_double), maxc, p(q, C.c_int32))
+        else:
+            H.hc_seam_wrap_flip_many(n, p(a, C.c_double), p(b, C.c_double), maxc, p(out, C.c_double),
+                                     p(dth, C.c_double), p(ok, C.c_int32), p(cr, C.c_int32), p(q, C.c_int32))
+        res.append((out, dth, ok, cr, q))
+    return res
+
+
+def test_seam_wrap_flip_form_bitwise(H):
+    """The lane-parallel loop's border tests and crossing (sign flips instead of selects) against
+    the direct form, bit for bit (zero signs included), on flights, edge and corner starts, exact
+    2.0 / 1.0 / -0.0 coordinates and near-corner passes."""
+    rng = np.random.default_rng(5)
+    n = 20000
+    a = rng.random((n, 2))
+    k = np.arange(n) % 10
+    a[k == 1, 0] = 0.0
+    a[k == 2, 1] = 0.0
+    a[k == 3, 0] = 1.0
+    a[k == 4, 1] = 1.0
+    a[k == 5] = 0.0
+    a[k == 6] = 1.0
+    a[k == 7] = (0.0, 1.0)
+    a[k == 8, 0] = -0.0
+    L = np.exp(rng.uniform(np.log(1e-4), np.log(80.0), n))
+    ang = rng.uniform(-np.pi, np.pi, n)
+    b = a + L[:, None] * np.stack([np.cos(ang), np.sin(ang)], 1)
+    # end points on exact values: 2.0, -1.0, 1.0, -0.0 in one coordinate
+    m = rng.random(n) < 0.15
+    b[m, rng.integers(0, 2)] = rng.choice([2.0, -1.0, 1.0, -0.0, 3.0], m.sum())
+    # near-corner passes (exit snapping) and moves along the edges
+    c = rng.choice([0.0, 1.0], (4000, 2))
+    off = rng.choice([1e-3, 9.99e-4, 1.001e-3, 1e-6, 0.0], (4000, 2)) * rng.choice([-1.0, 1.0], (4000, 2))
+    st = rng.random((4000, 2))
+    st[:1000, 0] = rng.choice([0.0, 1.0], 1000)
+    b2 = st + ((c + off) - st) * rng.uniform(1.0, 30.0, (4000, 1))
+    b2[:500, 0] = st[:500, 0]             # along an edge
+    A = np.concatenate([a, st]); B = np.concatenate([b, b2])
+    keep = ~(((B >= 0) & (B <= 1)).all(1))
+    (o1, t1, k1, c1, q1), (o2, t2, k2, c2, q2) = wrap_flip_vs_direct(H, A[keep], B[keep], 4096)
+    assert np.array_equal(k1, k2) and np.array_equal(c1, c2) and np.array_equal(q1, q2)
+    assert np.array_equal(o1.view(np.uint64), o2.view(np.uint64))
+    assert np.array_equal(t1.view(np.uint64), t2.view(np.uint64))
+    assert c1.max() > 40 and (c1 > 0).sum() > 10000
+
+
 def test_seam_wrap_reference_vector(H, O):
     import json
     g = json.load(open(os.path.join(HERE, "golden", "reference_vectors.json")))["seam_wrap"]
