@@ -255,7 +255,7 @@ void Engine::bucket_state_into(const int* v, BucketSet& b) {
   if (!d_tiles) ensure_tiles(resolve_tiled());
   clear_bucket_set(b);
   HIP_OK(launch_insert(ntot, v, d_rid, d_reps, md.nv, b, stream));
-  HIP_OK(launch_bucket_finalize(std::get<1>(tiles_key), d_reps, d_tiles, n_phys, md.nv, b, stream));
+  HIP_OK(launch_bucket_finalize(std::get<1>(tiles_key), d_reps, d_tiles, n_phys, md.nv, R == 1 ? ntot : 0, b, stream));
 }
 
 void Engine::ensure_buckets() {
@@ -484,7 +484,7 @@ void Engine::enqueue(const Shape& s, int n_steps, const float* nz, size_t stride
         if (rec)
           HIP_OK(launch_record(ntot, d_row0, t, rec->every, rec->depth, io.xycs_out, r3x, r3y, r3z, nbr, rec->d_ring,
                                rec->d_ring_nbr, stream));
-        HIP_OK(launch_bucket_finalize(s.block, d_reps, d_tiles, n_phys, md.nv, bs[(bp + 1) % 3], stream));
+        HIP_OK(launch_bucket_finalize(s.block, d_reps, d_tiles, n_phys, md.nv, R == 1 ? ntot : 0, bs[(bp + 1) % 3], stream));
         sp = 1 - sp;
         bp = (bp + 1) % 3;
       }

@@ -171,7 +171,7 @@ int32_t Engine::adjoint(int T, const float* nz, size_t stride, int K, const tiss
       io.tape_q = a.tp_q + q * N;
       HIP_OK(launch_step_tiled(s.lanes, s.block, true, md, d_reps, d_tiles, n_phys, set(q), set(q + 1), set(q + 2), io,
                                stream));
-      HIP_OK(launch_bucket_finalize(s.block, d_reps, d_tiles, n_phys, md.nv, set(q + 1), stream));
+      HIP_OK(launch_bucket_finalize(s.block, d_reps, d_tiles, n_phys, md.nv, R == 1 ? ntot : 0, set(q + 1), stream));
     }
     // backwards through the segment
     for (int q = kc - 1; q >= 0; --q) {

@@ -150,25 +150,38 @@ constexpr int kSortCap = 512;
 // A crowded bucket (more than kSlots) is then taken by its whole wave: its slots and its entries
 // of the overflow list are gathered (ballot compaction), ranked, and written in ascending order to
 // a fresh range of the replica's big area, whose offset replaces the bucket's first slot.
+// The kernel is a chain of dependent loads, so independent ones are issued together: the list
+// length, the overflow count and the thread's list entry (read speculatively within the
+// replica's range), then a bucket's count and all of its slots, and for a crowded bucket the
+// big-area reservation with the overflow scan. single_n > 0: one replica of single_n particles,
+// whose tiles need not be read (block b covers list entries b * blockDim.x ...).
 __global__ void __launch_bounds__(kStepMaxBlock) k_bucket_finalize(const ReplicaDesc* reps, const TileDesc* tiles,
-                                                                   int nv, BucketSet bs) {
-  __shared__ int lds[kStepMaxBlock / 64][kSortCap];
-  const TileDesc tile = tiles[blockIdx.x];
-  if (tile.r < 0) return;
-  const int r = tile.r;
-  const int off = reps[r].off;
+                                                                   int nv, int single_n, BucketSet bs) {
+  __shared__ __align__(16) int lds[kStepMaxBlock / 64][kSortCap];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int r = 0, off = 0, k = blockIdx.x * (int)blockDim.x + tid, cap = single_n;
+  if (single_n <= 0) {
+    const TileDesc tile = tiles[blockIdx.x];
+    if (tile.r < 0) return;
+    r = tile.r;
+    off = reps[r].off;
+    cap = reps[r].n;
+    k = tile.q * (int)blockDim.x + tid;
+  }
+  const int n_multi = bs.multi_n[r];
+  const int novf = bs.ovf_n[r];
+  const int v_listed = k < cap ? bs.multi[off + k] : -1;
   int* slot = bs.slot + (size_t)r * nv * kSlots;
-  const int k = tile.q * (int)blockDim.x + tid;
   int v = -1, c = 0;
-  if (k < bs.multi_n[r]) {
-    v = bs.multi[off + k];
+  if (k < n_multi) {
+    v = v_listed;
+    int* sl = slot + (size_t)v * kSlots;
+    const int4 s0 = reinterpret_cast<const int4*>(sl)[0], s1 = reinterpret_cast<const int4*>(sl)[1];
     c = bs.count[(size_t)r * nv + v];
     if (c <= kSlots) {
-      int* sl = slot + (size_t)v * kSlots;
-      int a[kSlots];
+      int a[kSlots] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
 #pragma unroll
-      for (int q = 0; q < kSlots; ++q) a[q] = q < c ? sl[q] : 0x7fffffff;
+      for (int q = 0; q < kSlots; ++q) a[q] = q < c ? a[q] : 0x7fffffff;
       sort8(a);
 #pragma unroll
       for (int q = 0; q < kSlots; ++q)
@@ -177,7 +190,6 @@ __global__ void __launch_bounds__(kStepMaxBlock) k_bucket_finalize(const Replica
   }
   unsigned long long crowded = __ballot(c > kSlots);
   if (crowded == 0ull) return;  // uniform over the wave
-  const int novf = bs.ovf_n[r];
   const int2* ovf = bs.ovf + off;
   int* big = bs.big + off;
   while (crowded) {
@@ -185,42 +197,64 @@ __global__ void __launch_bounds__(kStepMaxBlock) k_bucket_finalize(const Replica
     crowded &= crowded - 1ull;
     const int w = __shfl(v, src, 64);
     const int cw = __shfl(c, src, 64);
-    int base = 0;
-    if (lane == 0) base = atomicAdd(&bs.big_n[r], cw);
-    base = __shfl(base, 0, 64);
+    // reserve the bucket's range first: the reservation is in flight during the scan below
+    int reserved = 0;
+    if (lane == 0) reserved = atomicAdd(&bs.big_n[r], cw);
     int* sl = slot + (size_t)w * kSlots;
-    int* dst = big + base;
-    const bool in_lds = cw <= kSortCap;
-    int* buf = in_lds ? lds[wave] : dst;
-    if (lane < kSlots) buf[lane] = sl[lane];
-    int fill = kSlots;
-    for (int q0 = 0; q0 < novf; q0 += 64) {
-      const int q = q0 + lane;
-      const int2 e = q < novf ? ovf[q] : make_int2(-1, 0);
-      const bool m = e.x == w;
-      const unsigned long long bm = __ballot(m);
-      if (m) buf[fill + __popcll(bm & ((1ull << lane) - 1ull))] = e.y;
-      fill += __popcll(bm);
-    }
-    wave_sync();
+    const bool in_lds = cw <= kSortCap - 4;  // room for the padding below
     if (in_lds) {
-      // rank of each id among the bucket's (distinct) ids = its position in ascending order
+      int* buf = lds[wave];
+      if (lane < kSlots) buf[lane] = sl[lane];
+      int fill = kSlots;
+      for (int q0 = 0; q0 < novf; q0 += 64) {
+        const int q = q0 + lane;
+        const int2 e = q < novf ? ovf[q] : make_int2(-1, 0);
+        const bool m = e.x == w;
+        const unsigned long long bm = __ballot(m);
+        if (m) buf[fill + __popcll(bm & ((1ull << lane) - 1ull))] = e.y;
+        fill += __popcll(bm);
+      }
+      if (lane < 4) buf[cw + lane] = 0x7fffffff;  // pad to a multiple of four (never below an id)
+      wave_sync();
+      int* dst = big + __shfl(reserved, 0, 64);
+      // rank of each id among the bucket's (distinct) ids = its position in ascending order; the
+      // ids are read four at a time, several reads in flight
+      const int cw4 = (cw + 3) & ~3;
       for (int q = lane; q < cw; q += 64) {
         const int val = buf[q];
         int rank = 0;
-        for (int j = 0; j < cw; ++j) rank += buf[j] < val ? 1 : 0;
+#pragma unroll 4
+        for (int j = 0; j < cw4; j += 4) {
+          const int4 b4 = *reinterpret_cast<const int4*>(buf + j);
+          rank += (int)(b4.x < val) + (int)(b4.y < val) + (int)(b4.z < val) + (int)(b4.w < val);
+        }
         dst[rank] = val;
       }
-    } else if (lane == 0) {
-      for (int a = 1; a < cw; ++a) {  // rare: insertion sort in place in global memory
-        const int val = dst[a];
-        int b = a - 1;
-        while (b >= 0 && dst[b] > val) { dst[b + 1] = dst[b]; --b; }
-        dst[b + 1] = val;
+    } else {
+      // rare: gathered and sorted in place in global memory
+      int* dst = big + __shfl(reserved, 0, 64);
+      if (lane < kSlots) dst[lane] = sl[lane];
+      int fill = kSlots;
+      for (int q0 = 0; q0 < novf; q0 += 64) {
+        const int q = q0 + lane;
+        const int2 e = q < novf ? ovf[q] : make_int2(-1, 0);
+        const bool m = e.x == w;
+        const unsigned long long bm = __ballot(m);
+        if (m) dst[fill + __popcll(bm & ((1ull << lane) - 1ull))] = e.y;
+        fill += __popcll(bm);
+      }
+      wave_sync();
+      if (lane == 0) {
+        for (int a = 1; a < cw; ++a) {
+          const int val = dst[a];
+          int b = a - 1;
+          while (b >= 0 && dst[b] > val) { dst[b + 1] = dst[b]; --b; }
+          dst[b + 1] = val;
+        }
       }
     }
     wave_sync();
-    if (lane == 0) sl[0] = base;
+    if (lane == 0) sl[0] = reserved;
   }
 }
 
@@ -388,8 +422,8 @@ hipError_t launch_step_tiled(int lanes, int block, bool tape, const MeshDev& mes
 }
 
 hipError_t launch_bucket_finalize(int block, const ReplicaDesc* reps, const TileDesc* tiles, int n_tiles, int nv,
-                                  const BucketSet& bs, hipStream_t st) {
-  hipLaunchKernelGGL(k_bucket_finalize, dim3(n_tiles), dim3(block), 0, st, reps, tiles, nv, bs);
+                                  int single_n, const BucketSet& bs, hipStream_t st) {
+  hipLaunchKernelGGL(k_bucket_finalize, dim3(n_tiles), dim3(block), 0, st, reps, tiles, nv, single_n, bs);
   return hipGetLastError();
 }
 

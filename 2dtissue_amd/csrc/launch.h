@@ -118,9 +118,10 @@ hipError_t launch_step_tiled(int lanes, int block, bool tape, const tdev::MeshDe
                              const TileDesc* tiles, int n_tiles, const BucketSet& bin, const BucketSet& bout,
                              const BucketSet& bclr, const TiledIO& io, hipStream_t st);
 // sort the buckets of set bs that hold two or more particles (over the same tiles as the step);
-// crowded buckets (more than kSlots) move, sorted, into the big area
+// crowded buckets (more than kSlots) move, sorted, into the big area. single_n: the particle
+// count when there is one replica (its tiles are then not read), else 0
 hipError_t launch_bucket_finalize(int block, const ReplicaDesc* reps, const TileDesc* tiles, int n_tiles, int nv,
-                                  const BucketSet& bs, hipStream_t st);
+                                  int single_n, const BucketSet& bs, hipStream_t st);
 // order parameters of `steps` steps of every replica from the tiles' partials
 // (partial[s * n_logical + tile]); replica r's tiles are tile_begin[r] .. tile_begin[r+1]
 hipError_t launch_phi_tiles(int R, int steps, const int* tile_begin, int n_logical, const OrderPartial* partial,
