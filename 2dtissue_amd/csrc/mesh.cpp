@@ -6,6 +6,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <fstream>
 #include <functional>
 #include <limits>
@@ -19,9 +20,43 @@
 
 namespace tissue {
 
+int build_threads();
+
 namespace {
 
 inline double f32(double x) { return (double)(float)x; }
+
+// Threads that run one function in lockstep: wait() returns once every member has reached it.
+// Members spin briefly, then yield (more members than free cores stays correct, only slower).
+class SpinBarrier {
+ public:
+  explicit SpinBarrier(int n) : n_(n) {}
+  void wait() {
+    const unsigned g = gen_.load(std::memory_order_acquire);
+    if (count_.fetch_add(1, std::memory_order_acq_rel) == n_ - 1) {
+      count_.store(0, std::memory_order_relaxed);
+      gen_.fetch_add(1, std::memory_order_acq_rel);
+      return;
+    }
+    for (int spin = 0; gen_.load(std::memory_order_acquire) == g; ++spin)
+      if (spin > 256) std::this_thread::yield();
+  }
+
+ private:
+  const int n_;
+  std::atomic<int> count_{0};
+  std::atomic<unsigned> gen_{0};
+};
+
+// run f(member, team size, barrier) on `threads` threads (the caller is member 0)
+template <class F>
+void run_team(int threads, F&& f) {
+  SpinBarrier bar(threads);
+  std::vector<std::thread> pool;
+  for (int k = 1; k < threads; ++k) pool.emplace_back([&, k] { f(k, threads, bar); });
+  f(0, threads, bar);
+  for (auto& t : pool) t.join();
+}
 
 inline void sub3(const double* a, const double* b, double* o) {
   o[0] = a[0] - b[0]; o[1] = a[1] - b[1]; o[2] = a[2] - b[2];
@@ -412,36 +447,141 @@ UVMesh cut_and_parameterize(const SurfaceMesh& m, int pa, int pm, int pb) {
     }
     rp[r + 1] = (int)ci.size();
   }
-  auto solve = [&](const std::vector<double>& b, std::vector<double>& x) {
-    x.assign(ni, 0.0);
-    std::vector<double> r = b, z(ni), p(ni), Ap(ni);
-    for (int i = 0; i < ni; ++i) { z[i] = r[i] / diag[i]; p[i] = z[i]; }
-    double rz = 0, bn = 0;
-    for (int i = 0; i < ni; ++i) { rz += r[i] * z[i]; bn += b[i] * b[i]; }
-    bn = std::sqrt(bn);
-    if (bn == 0) return;
-    for (int it = 0; it < 20 * ni + 100; ++it) {
-      for (int i = 0; i < ni; ++i) {
-        double a = diag[i] * p[i];
-        for (int e = rp[i]; e < rp[i + 1]; ++e) a += val[e] * p[ci[e]];
-        Ap[i] = a;
-      }
-      double pAp = 0;
-      for (int i = 0; i < ni; ++i) pAp += p[i] * Ap[i];
-      double alpha = rz / pAp, rn = 0;
-      for (int i = 0; i < ni; ++i) { x[i] += alpha * p[i]; r[i] -= alpha * Ap[i]; rn += r[i] * r[i]; }
-      if (std::sqrt(rn) <= 1e-13 * bn) return;
-      double rz2 = 0;
-      for (int i = 0; i < ni; ++i) { z[i] = r[i] / diag[i]; rz2 += r[i] * z[i]; }
-      double beta = rz2 / rz;
-      rz = rz2;
-      for (int i = 0; i < ni; ++i) p[i] = z[i] + beta * p[i];
+  // Jacobi-preconditioned CG for both coordinates at once (the two systems share the matrix),
+  // rows split into fixed blocks of kBlock: every dot product is the sum of per-block partials
+  // (rows in order within a block) added in block order, so the solution does not depend on the
+  // number of threads. Threads take blocks round-robin and meet at a barrier after each of the
+  // three phases of an iteration; every thread forms the same scalars from the partials and so
+  // takes the same convergence decisions. A coordinate stops updating once converged (each
+  // coordinate's arithmetic is that of its own solve).
+  constexpr int kBlock = 2048;
+  const int nb = (ni + kBlock - 1) / kBlock;
+  const std::vector<double>* bs[2] = {&bx, &by};
+  std::vector<double> sol[2], res[2], zz[2], pp[2], Ap[2];
+  std::vector<double> part(6 * (size_t)nb);  // per block: pAp, r.r, r.z for x and y
+  for (int k = 0; k < 2; ++k) {
+    sol[k].assign(ni, 0.0);
+    res[k] = *bs[k];
+    zz[k].assign(ni, 0.0);
+    pp[k].assign(ni, 0.0);
+    Ap[k].assign(ni, 0.0);
+  }
+  double rz[2] = {0, 0}, bn[2] = {0, 0};
+  bool live[2] = {false, false};
+  for (int k = 0; k < 2; ++k) {
+    for (int i = 0; i < ni; ++i) {
+      zz[k][i] = res[k][i] / diag[i];
+      pp[k][i] = zz[k][i];
     }
-    throw std::runtime_error("cut: UV solve did not converge");
-  };
-  std::vector<double> sx, sy;
-  solve(bx, sx);
-  solve(by, sy);
+    for (int i = 0; i < ni; ++i) {
+      rz[k] += res[k][i] * zz[k][i];
+      bn[k] += (*bs[k])[i] * (*bs[k])[i];
+    }
+    bn[k] = std::sqrt(bn[k]);
+    live[k] = bn[k] != 0;
+  }
+  bool converged = !live[0] && !live[1];
+  if (!converged) {
+    const int max_it = 20 * ni + 100;
+    const int threads = std::min(build_threads(), nb);
+    run_team(threads, [&](int me, int nt, SpinBarrier& bar) {
+      double rzl[2] = {rz[0], rz[1]};
+      bool on[2] = {live[0], live[1]};
+      auto total = [&](int slot) {
+        double t = 0;
+        for (int q = 0; q < nb; ++q) t += part[6 * (size_t)q + slot];
+        return t;
+      };
+      for (int it = 0; it < max_it; ++it) {
+        // (1) Ap = A p and the partials of p.Ap (one pass over the matrix for both coordinates)
+        for (int q = me; q < nb; q += nt) {
+          const int i0 = q * kBlock, i1 = std::min(ni, i0 + kBlock);
+          if (on[0] && on[1]) {
+            const double *p0 = pp[0].data(), *p1 = pp[1].data();
+            double acc0 = 0, acc1 = 0;
+            for (int i = i0; i < i1; ++i) {
+              double a0 = diag[i] * p0[i], a1 = diag[i] * p1[i];
+              for (int e = rp[i]; e < rp[i + 1]; ++e) {
+                a0 += val[e] * p0[ci[e]];
+                a1 += val[e] * p1[ci[e]];
+              }
+              Ap[0][i] = a0;
+              Ap[1][i] = a1;
+              acc0 += p0[i] * a0;
+              acc1 += p1[i] * a1;
+            }
+            part[6 * (size_t)q] = acc0;
+            part[6 * (size_t)q + 1] = acc1;
+            continue;
+          }
+          for (int k = 0; k < 2; ++k) {
+            if (!on[k]) continue;
+            const double* pk = pp[k].data();
+            double* ak = Ap[k].data();
+            double acc = 0;
+            for (int i = i0; i < i1; ++i) {
+              double a = diag[i] * pk[i];
+              for (int e = rp[i]; e < rp[i + 1]; ++e) a += val[e] * pk[ci[e]];
+              ak[i] = a;
+              acc += pk[i] * a;
+            }
+            part[6 * (size_t)q + k] = acc;
+          }
+        }
+        bar.wait();
+        double alpha[2] = {0, 0};
+        for (int k = 0; k < 2; ++k)
+          if (on[k]) alpha[k] = rzl[k] / total(k);
+        // (2) x += alpha p, r -= alpha Ap, z = r / diag; partials of r.r and r.z
+        for (int q = me; q < nb; q += nt) {
+          const int i0 = q * kBlock, i1 = std::min(ni, i0 + kBlock);
+          for (int k = 0; k < 2; ++k) {
+            if (!on[k]) continue;
+            double rn = 0, rzn = 0;
+            for (int i = i0; i < i1; ++i) {
+              sol[k][i] += alpha[k] * pp[k][i];
+              res[k][i] -= alpha[k] * Ap[k][i];
+              rn += res[k][i] * res[k][i];
+            }
+            for (int i = i0; i < i1; ++i) {
+              zz[k][i] = res[k][i] / diag[i];
+              rzn += res[k][i] * zz[k][i];
+            }
+            part[6 * (size_t)q + 2 + k] = rn;
+            part[6 * (size_t)q + 4 + k] = rzn;
+          }
+        }
+        bar.wait();
+        double beta[2] = {0, 0};
+        for (int k = 0; k < 2; ++k) {
+          if (!on[k]) continue;
+          if (std::sqrt(total(2 + k)) <= 1e-13 * bn[k]) {
+            on[k] = false;
+            continue;
+          }
+          const double rz2 = total(4 + k);
+          beta[k] = rz2 / rzl[k];
+          rzl[k] = rz2;
+        }
+        if (!on[0] && !on[1]) {
+          if (me == 0) converged = true;
+          return;
+        }
+        // (3) p = z + beta p (the partial slots of phases 1 and 2 are disjoint, and a barrier
+        // separates each phase's writes from the previous reads of its slots)
+        for (int q = me; q < nb; q += nt) {
+          const int i0 = q * kBlock, i1 = std::min(ni, i0 + kBlock);
+          for (int k = 0; k < 2; ++k)
+            if (on[k])
+              for (int i = i0; i < i1; ++i) pp[k][i] = zz[k][i] + beta[k] * pp[k][i];
+        }
+        bar.wait();
+      }
+    });
+  }
+  if (!converged) throw std::runtime_error("cut: UV solve did not converge");
+  const std::vector<double>& sx = sol[0];
+  const std::vector<double>& sy = sol[1];
   for (int v = 0; v < nv; ++v)
     if (!fixed[v]) {
       uv[2 * v] = f32(std::clamp(sx[idx[v]], 0.0, 1.0));
@@ -551,8 +691,14 @@ double unfold_across(const double* V, const double* A, const double* B, const do
 
 }  // namespace
 
-// worker threads of the table build: the host's cores, at most 16
+// worker threads of the mesh build (UV solve, geodesic table): the host's cores, at most 16, or
+// the environment variable TISSUE_HOST_THREADS when set to a positive count (at most 64); the
+// results do not depend on it
 int build_threads() {
+  if (const char* e = std::getenv("TISSUE_HOST_THREADS")) {
+    const int t = std::atoi(e);
+    if (t > 0) return std::min(t, 64);
+  }
   const unsigned h = std::thread::hardware_concurrency();
   return (int)std::clamp<unsigned>(h == 0 ? 1u : h, 1u, 16u);
 }

@@ -120,6 +120,28 @@ def test_distance_table_structure(ell):
         assert lookup[(w, v)] == d              # symmetric (min of both directions)
 
 
+def test_mesh_build_independent_of_thread_count(tmp_path):
+    """The UV solve and the geodesic table are split over host threads; their results must not
+    depend on how many: the same mesh built with 1 and with 5 threads, byte for byte."""
+    import subprocess
+    import sys
+    code = (
+        "import importlib, sys, numpy as np\n"
+        f"sys.path.insert(0, {ROOT!r})\n"
+        "T = importlib.import_module('2dtissue_amd')\n"
+        "m = T.Mesh.ellipsoid(2.0, 1.0, 1.0, 28).build_tables(0.1)\n"
+        "e = m.export()\n"
+        "np.savez(sys.argv[1], **{k: e[k] for k in ('uv', 'xyz', 'faces', 'row_ptr', 'col', 'dist', 'cell_start', 'cell_faces')})\n")
+    out = {}
+    for n in (1, 5):
+        f = str(tmp_path / f"mesh{n}.npz")
+        env = dict(os.environ, TISSUE_HOST_THREADS=str(n))
+        subprocess.run([sys.executable, "-c", code, f], check=True, env=env, timeout=300)
+        out[n] = np.load(f)
+    for k in out[1].files:
+        assert out[1][k].tobytes() == out[5][k].tobytes(), k
+
+
 def test_geodesic_table_vs_chord_and_sphere(T):
     """On the unit sphere the table must lie between the chord (a lower bound) and the
     great-circle distance plus the mesh's discretisation error."""
