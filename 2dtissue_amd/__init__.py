@@ -23,6 +23,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtissue.so")
 
 OK = 0
+BUILD_AUTO, BUILD_HOST, BUILD_GPU = -1, 0, 1
 E_INVALID_ARG = 1
 E_LOST_PARTICLE = 2
 E_NONFINITE = 3
@@ -113,6 +114,7 @@ SIGNATURES = [
     ("tissue_mesh_perturbed_sphere", C.c_int32, [C.c_double, C.c_int32, C.POINTER(_P)]),
     ("tissue_mesh_load_off", C.c_int32, [C.c_char_p, C.POINTER(_P)]),
     ("tissue_mesh_build_tables", C.c_int32, [_P, C.c_double, C.c_int32]),
+    ("tissue_mesh_build_tables_on", C.c_int32, [_P, C.c_double, C.c_int32, C.c_int32]),
     ("tissue_mesh_destroy", None, [_P]),
     ("tissue_mesh_info", C.c_int32, [_P, _I, _I, _I, C.POINTER(C.c_int64), _I, C.POINTER(C.c_int64)]),
     ("tissue_mesh_export", C.c_int32, [_P, _D, _D, _I, _I, _I, _F, _I, _I]),
@@ -218,8 +220,13 @@ class Mesh:
         _check(lib().tissue_mesh_load_off(path.encode(), C.byref(h)))
         return cls(h)
 
-    def build_tables(self, sigma_max: float, grid: int = 0) -> "Mesh":
-        _check(lib().tissue_mesh_build_tables(self.h, sigma_max, grid))
+    def build_tables(self, sigma_max: float, grid: int = 0, where: int | None = None) -> "Mesh":
+        """Face buckets and geodesic table (tissue_mesh_build_tables); where = BUILD_HOST or
+        BUILD_GPU places the table's marches (tissue_mesh_build_tables_on), None: automatic."""
+        if where is None:
+            _check(lib().tissue_mesh_build_tables(self.h, sigma_max, grid))
+        else:
+            _check(lib().tissue_mesh_build_tables_on(self.h, sigma_max, grid, where))
         return self
 
     def info(self) -> dict:

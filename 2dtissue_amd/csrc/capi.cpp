@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "capi_common.h"
+#include "mesh_gpu.h"
 #include "rng.h"
 
 using namespace tcapi;
@@ -94,17 +95,31 @@ int32_t tissue_mesh_load_off(const char* path, tissue_mesh** out) {
   });
 }
 
-int32_t tissue_mesh_build_tables(tissue_mesh* m, double sigma_max, int32_t grid) {
+int32_t tissue_mesh_build_tables_on(tissue_mesh* m, double sigma_max, int32_t grid, int32_t where) {
   return guarded([&] {
     if (!m || !(sigma_max > 0) || !std::isfinite(sigma_max)) throw ApiError(TISSUE_E_INVALID_ARG, "bad table arguments");
+    if (where < TISSUE_BUILD_AUTO || where > TISSUE_BUILD_GPU) throw ApiError(TISSUE_E_INVALID_ARG, "bad build place");
     std::lock_guard<std::mutex> lk(m->mu);
     if (m->dev_ready) throw ApiError(TISSUE_E_INVALID_ARG, "mesh tables are in use on a device");
+    int ndev = 0;
+    const bool have_gpu = hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0;
+    if (where == TISSUE_BUILD_GPU && !have_gpu) throw ApiError(TISSUE_E_INVALID_ARG, "no GPU for the table build");
+    const bool gpu = where == TISSUE_BUILD_GPU || (where == TISSUE_BUILD_AUTO && have_gpu);
     tissue::build_face_buckets(m->m, grid);
-    tissue::build_distance_table(m->m, 2.4 * sigma_max);
+    try {
+      tissue::build_distance_table(m->m, 2.4 * sigma_max, gpu ? tissue::DeviceMarch(tissue::march_on_gpu) : nullptr);
+    } catch (const std::runtime_error& e) {
+      if (gpu) throw ApiError(TISSUE_E_HIP, e.what());
+      throw;
+    }
     m->sigma_max = sigma_max;
     m->tables = true;
     return TISSUE_OK;
   });
+}
+
+int32_t tissue_mesh_build_tables(tissue_mesh* m, double sigma_max, int32_t grid) {
+  return tissue_mesh_build_tables_on(m, sigma_max, grid, TISSUE_BUILD_AUTO);
 }
 
 void tissue_mesh_destroy(tissue_mesh* m) {

@@ -1,6 +1,8 @@
 // Mesh toolkit (host, FP64). See mesh.h for what each table replaces in the reference.
 #include "mesh.h"
 
+#include "geodesic.h"
+
 #include <algorithm>
 #include <array>
 #include <atomic>
@@ -646,49 +648,6 @@ void build_face_buckets(UVMesh& m, int grid) {
 
 namespace {
 
-// first-order fast-marching update of vertex C from triangle (A, B, C) with known arrival
-// times tA, tB (Kimmel & Sethian 1998, written for 3D edge vectors): the planar front
-// T(P) = tC + g.(P - C), |g| = 1, through A and B; accepted only when the characteristic
-// reaching C enters the triangle through edge AB.
-double tri_update(const double* A, double tA, const double* B, double tB, const double* C) {
-  double e1[3], e2[3];
-  sub3(A, C, e1);
-  sub3(B, C, e2);
-  const double m00 = dot3(e1, e1), m01 = dot3(e1, e2), m11 = dot3(e2, e2);
-  const double det = m00 * m11 - m01 * m01;
-  if (!(det > 1e-300)) return std::numeric_limits<double>::infinity();
-  const double i00 = m11 / det, i01 = -m01 / det, i11 = m00 / det;
-  const double a = i00 + 2 * i01 + i11;
-  const double b = (i00 + i01) * tA + (i01 + i11) * tB;
-  const double c = i00 * tA * tA + 2 * i01 * tA * tB + i11 * tB * tB - 1.0;
-  const double disc = b * b - a * c;
-  if (!(a > 0) || disc < 0) return std::numeric_limits<double>::infinity();
-  const double T = (b + std::sqrt(disc)) / a;
-  const double r0 = tA - T, r1 = tB - T;
-  const double c0 = i00 * r0 + i01 * r1, c1 = i01 * r0 + i11 * r1;
-  const double tol = 1e-12;
-  if (c0 > tol || c1 > tol || T < std::max(tA, tB)) return std::numeric_limits<double>::infinity();
-  return T;
-}
-
-// length of the straight path from V to X across edge (A, B) once the two faces (V,A,B) and
-// (A,B,X) are unfolded into one plane; infinity when that segment misses the edge
-double unfold_across(const double* V, const double* A, const double* B, const double* X) {
-  double ab[3], av[3], ax[3];
-  sub3(B, A, ab);
-  sub3(V, A, av);
-  sub3(X, A, ax);
-  const double L = norm3(ab);
-  if (!(L > 0)) return std::numeric_limits<double>::infinity();
-  const double vu = dot3(av, ab) / L, xu = dot3(ax, ab) / L;
-  const double vh = std::sqrt(std::max(0.0, dot3(av, av) - vu * vu));
-  const double xh = std::sqrt(std::max(0.0, dot3(ax, ax) - xu * xu));
-  if (!(vh + xh > 0)) return std::numeric_limits<double>::infinity();
-  const double t = vu + (xu - vu) * vh / (vh + xh);   // crossing point along AB
-  if (t < 0 || t > L) return std::numeric_limits<double>::infinity();
-  return std::sqrt((xu - vu) * (xu - vu) + (xh + vh) * (xh + vh));
-}
-
 }  // namespace
 
 // worker threads of the mesh build (UV solve, geodesic table): the host's cores, at most 16, or
@@ -703,7 +662,7 @@ int build_threads() {
   return (int)std::clamp<unsigned>(h == 0 ? 1u : h, 1u, 16u);
 }
 
-void build_distance_table(UVMesh& m, double cutoff) {
+void build_distance_table(UVMesh& m, double cutoff, const DeviceMarch& device_march) {
   if (!(cutoff > 0)) throw std::runtime_error("distance table: cutoff must be positive");
   const int nv = m.nv;
   const double lim = cutoff * (1.0 + 1e-6);
@@ -711,11 +670,46 @@ void build_distance_table(UVMesh& m, double cutoff) {
   for (int f = 0; f < m.nf; ++f)
     for (int q = 0; q < 3; ++q) vf[m.faces[3 * f + q]].push_back(f);
 
-  std::unordered_map<int64_t, std::vector<int>> ef;
+  // the faces on each edge: (edge key, face) pairs sorted, grouped by edge (faces ascending);
+  // edge_of[3 f + q] is the group of the edge opposite corner q of face f
   auto edge_key = [](int a, int b) { return a < b ? ((int64_t)a << 32) | b : ((int64_t)b << 32) | a; };
+  std::vector<std::pair<int64_t, int>> ek(3 * (size_t)m.nf);
   for (int f = 0; f < m.nf; ++f)
-    for (int q = 0; q < 3; ++q) ef[edge_key(m.faces[3 * f + q], m.faces[3 * f + (q + 1) % 3])].push_back(f);
-  std::vector<std::vector<std::pair<int, double>>> rows(nv);
+    for (int q = 0; q < 3; ++q)
+      ek[3 * (size_t)f + q] = {edge_key(m.faces[3 * f + (q + 1) % 3], m.faces[3 * f + (q + 2) % 3]), 3 * f + q};
+  std::sort(ek.begin(), ek.end());
+  std::vector<int32_t> edge_ptr{0}, edge_faces, edge_of(3 * (size_t)m.nf);
+  edge_faces.reserve(ek.size());
+  bool manifold = true;
+  for (size_t k = 0; k < ek.size();) {
+    size_t e = k;
+    for (; e < ek.size() && ek[e].first == ek[k].first; ++e) {
+      edge_faces.push_back(ek[e].second / 3);
+      edge_of[ek[e].second] = (int32_t)edge_ptr.size() - 1;
+    }
+    if (e - k > 2) manifold = false;
+    edge_ptr.push_back((int32_t)edge_faces.size());
+    k = e;
+  }
+  MarchRows rows(nv);
+  std::vector<char> done(nv, 0);
+  if (device_march && manifold) {
+    // the device takes the sources first (manifold meshes: at most one face across an edge);
+    // the host marches whatever it leaves
+    MarchMesh mm;
+    mm.vf_ptr.assign(nv + 1, 0);
+    for (int v = 0; v < nv; ++v) mm.vf_ptr[v + 1] = mm.vf_ptr[v] + (int32_t)vf[v].size();
+    mm.vf_idx.reserve(mm.vf_ptr[nv]);
+    for (int v = 0; v < nv; ++v) mm.vf_idx.insert(mm.vf_idx.end(), vf[v].begin(), vf[v].end());
+    mm.across.assign(3 * (size_t)m.nf, -1);
+    for (int f = 0; f < m.nf; ++f)
+      for (int q = 0; q < 3; ++q) {
+        const int g = edge_of[3 * (size_t)f + q];
+        for (int k = edge_ptr[g]; k < edge_ptr[g + 1]; ++k)
+          if (edge_faces[k] != f) mm.across[3 * (size_t)f + q] = edge_faces[k];
+      }
+    device_march(m, mm, lim, rows, done);
+  }
   using QE = std::pair<double, int>;
   // one local march per source vertex; sources are independent, so threads take them in turn,
   // each with its own arrival times (the rows come out the same whatever the thread count)
@@ -725,6 +719,7 @@ void build_distance_table(UVMesh& m, double cutoff) {
     std::vector<char> alive(nv, 0);
     std::vector<int> touched;
     for (int s = next_source++; s < nv; s = next_source++) {
+      if (done[s]) continue;
       std::priority_queue<QE, std::vector<QE>, std::greater<QE>> pq;
       auto relax = [&](int tgt, double cand) {
         if (cand < T[tgt]) {
@@ -751,20 +746,23 @@ void build_distance_table(UVMesh& m, double cutoff) {
             if (tgt == v || alive[tgt]) continue;
             const int oth = fc[0] + fc[1] + fc[2] - v - tgt;
             double d[3];
-            sub3(&m.xyz[3 * tgt], X, d);
-            double cand = tv + norm3(d);
-            if (alive[oth]) cand = std::min(cand, tri_update(X, tv, &m.xyz[3 * oth], T[oth], &m.xyz[3 * tgt]));
+            geo::sub3(&m.xyz[3 * tgt], X, d);
+            double cand = tv + geo::norm3(d);
+            if (alive[oth]) cand = std::min(cand, geo::tri_update(X, tv, &m.xyz[3 * oth], T[oth], &m.xyz[3 * tgt]));
             relax(tgt, cand);
           }
           // straight segment from v across the opposite edge (a, b) into the neighbouring face
           const int a = fc[0] == v ? fc[1] : fc[0];
           const int b = fc[0] + fc[1] + fc[2] - v - a;
-          for (int g2 : ef.at(edge_key(a, b))) {
+          const int qv = fc[0] == v ? 0 : (fc[1] == v ? 1 : 2);
+          const int eg = edge_of[3 * (size_t)f + qv];
+          for (int k = edge_ptr[eg]; k < edge_ptr[eg + 1]; ++k) {
+            const int g2 = edge_faces[k];
             if (g2 == f) continue;
             const int32_t* gc = &m.faces[3 * g2];
             const int x = gc[0] + gc[1] + gc[2] - a - b;
             if (alive[x]) continue;
-            double L = unfold_across(X, &m.xyz[3 * a], &m.xyz[3 * b], &m.xyz[3 * x]);
+            double L = geo::unfold_across(X, &m.xyz[3 * a], &m.xyz[3 * b], &m.xyz[3 * x]);
             relax(x, tv + L);
           }
         }
