@@ -242,20 +242,23 @@ size_t Engine::bucket_set_bytes() const {
          3 * (size_t)R * sizeof(int);
 }
 
-// outside any capture: clear set b (count and list lengths)
-void Engine::clear_bucket_set(BucketSet& b) {
-  HIP_OK(hipMemsetAsync(b.count, 0, (size_t)R * md.nv * sizeof(int), stream));
-  HIP_OK(hipMemsetAsync(b.ovf_n, 0, R * sizeof(int), stream));
-  HIP_OK(hipMemsetAsync(b.multi_n, 0, R * sizeof(int), stream));
-  HIP_OK(hipMemsetAsync(b.big_n, 0, R * sizeof(int), stream));
+// outside any capture: clear set b (count and list lengths), on stream st (null: the engine's)
+void Engine::clear_bucket_set(BucketSet& b, hipStream_t st) {
+  if (!st) st = stream;
+  HIP_OK(hipMemsetAsync(b.count, 0, (size_t)R * md.nv * sizeof(int), st));
+  HIP_OK(hipMemsetAsync(b.ovf_n, 0, R * sizeof(int), st));
+  HIP_OK(hipMemsetAsync(b.multi_n, 0, R * sizeof(int), st));
+  HIP_OK(hipMemsetAsync(b.big_n, 0, R * sizeof(int), st));
 }
 
-// outside any capture: clear set b, bucket the vertex ids v into it and sort its buckets
-void Engine::bucket_state_into(const int* v, BucketSet& b) {
+// outside any capture: clear set b, bucket the vertex ids v into it and sort its buckets, on
+// stream st (null: the engine's)
+void Engine::bucket_state_into(const int* v, BucketSet& b, hipStream_t st) {
+  if (!st) st = stream;
   if (!d_tiles) ensure_tiles(resolve_tiled());
-  clear_bucket_set(b);
-  HIP_OK(launch_insert(ntot, v, d_rid, d_reps, md.nv, b, stream));
-  HIP_OK(launch_bucket_finalize(std::get<1>(tiles_key), d_reps, d_tiles, n_phys, md.nv, R == 1 ? ntot : 0, b, stream));
+  clear_bucket_set(b, st);
+  HIP_OK(launch_insert(ntot, v, d_rid, d_reps, md.nv, b, st));
+  HIP_OK(launch_bucket_finalize(std::get<1>(tiles_key), d_reps, d_tiles, n_phys, md.nv, R == 1 ? ntot : 0, b, st));
 }
 
 void Engine::ensure_buckets() {

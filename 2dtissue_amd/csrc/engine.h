@@ -124,13 +124,23 @@ struct Engine {
 
   std::map<std::tuple<int, const float*, size_t, std::tuple<int, int, int, int, int>, int, int>, hipGraphExec_t> graphs;
 
-  // adjoint buffers (grown on demand): checkpoints [C][ntot], the segment tape [K+1][ntot] of
-  // states and [K][ntot] of force / alignment sums and seam-wrap turns, adjoint state, factors
+  // one segment's tape: states [K+1][ntot], force / alignment sums and seam-wrap turns [K][ntot],
+  // and the bucket sets of its states when they fit (else empty: each reverse step re-buckets)
+  struct Tape {
+    float4* xycs = nullptr; float* th = nullptr; int* vid = nullptr;
+    float4* fs = nullptr; int* q = nullptr;
+    std::vector<BucketSet> sbs;
+  };
+  // adjoint buffers (grown on demand): checkpoints [C][ntot], tapes, adjoint state, factors.
+  // With two tapes the recompute of a segment (on rec_stream) runs while the reverse pass of the
+  // following segment (on the engine's stream) runs; events order the tape hand-overs.
   struct AdjointBuffers {
     int K = 0, C = 0;
     float4* ck_xycs = nullptr; float* ck_th = nullptr; int* ck_vid = nullptr;
-    float4* tp_xycs = nullptr; float* tp_th = nullptr; int* tp_vid = nullptr;
-    float4* tp_fs = nullptr; int* tp_q = nullptr;
+    Tape tp[2];
+    int ntape = 0;
+    hipStream_t rec_stream = nullptr;
+    hipEvent_t ev_fwd = nullptr, ev_rec[2] = {nullptr, nullptr}, ev_rev[2] = {nullptr, nullptr};
     float2* lr[2] = {nullptr, nullptr}; float* lt[2] = {nullptr, nullptr};
     float2* gF = nullptr; float2* gSC = nullptr; float4* part = nullptr;
     double* acc = nullptr; double* out_seed = nullptr; double* out_red = nullptr;
@@ -140,8 +150,6 @@ struct Engine {
     int* sexp = nullptr;  // [2][R]: the adjoint state is stored scaled by 2^-sexp (slots alternate by step)
     unsigned* mx = nullptr;  // [2][R]: max |lambda| of the last gather's output (float bits)
     OrderPartial* partial = nullptr; int partial_n = 0;
-    // bucket sets of every state of a segment (kept for the reverse steps), when they fit
-    std::vector<BucketSet> sbs;
   } adj;
   void adjoint_buffers(int K, int C);
   void adjoint_release();
@@ -151,10 +159,10 @@ struct Engine {
   void ensure_buckets();
   void rebuild_buckets();
   void bucket_state(const int* vid, int set);
-  void bucket_state_into(const int* vid, BucketSet& b);
+  void bucket_state_into(const int* vid, BucketSet& b, hipStream_t st = nullptr);
   void alloc_bucket_set(BucketSet& b);
   void free_bucket_set(BucketSet& b);
-  void clear_bucket_set(BucketSet& b);
+  void clear_bucket_set(BucketSet& b, hipStream_t st = nullptr);
   size_t bucket_set_bytes() const;
   void ensure_tiles(const Shape& s);
   void enqueue(const Shape& s, int n_steps, const float* noise, size_t stride);

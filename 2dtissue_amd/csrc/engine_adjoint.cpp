@@ -7,7 +7,8 @@
 // buckets of every state of the segment (when they fit in about 16 GB; otherwise each reverse
 // step rebuilds its state's buckets from the recorded vertex ids); then, step by step backwards,
 // the two reverse kernels (per-particle factors, then the transposed neighbour gather over the
-// step's buckets) map the adjoint of the new state to the adjoint of the old one.
+// step's buckets) map the adjoint of the new state to the adjoint of the old one. With two tapes
+// the recompute of the previous segment runs on a second stream during that reverse pass.
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -48,13 +49,19 @@ LossDev make_loss(const tissue_loss& l, int n_steps) {
 
 void Engine::adjoint_release() {
   AdjointBuffers& a = adj;
+  if (a.rec_stream) (void)hipStreamSynchronize(a.rec_stream);
   dfree(a.ck_xycs); dfree(a.ck_th); dfree(a.ck_vid);
-  dfree(a.tp_xycs); dfree(a.tp_th); dfree(a.tp_vid); dfree(a.tp_fs); dfree(a.tp_q);
+  for (Tape& t : a.tp) {
+    dfree(t.xycs); dfree(t.th); dfree(t.vid); dfree(t.fs); dfree(t.q);
+    for (BucketSet& b : t.sbs) free_bucket_set(b);
+  }
   for (int k = 0; k < 2; ++k) { dfree(a.lr[k]); dfree(a.lt[k]); }
   dfree(a.gF); dfree(a.gSC); dfree(a.part); dfree(a.acc); dfree(a.out_seed); dfree(a.out_red);
   dfree(a.vx); dfree(a.vy); dfree(a.r3x); dfree(a.r3y); dfree(a.r3z); dfree(a.face); dfree(a.nbr);
   dfree(a.rows); dfree(a.first_bad); dfree(a.partial); dfree(a.sexp); dfree(a.mx);
-  for (BucketSet& b : a.sbs) free_bucket_set(b);
+  for (hipEvent_t e : {a.ev_fwd, a.ev_rec[0], a.ev_rec[1], a.ev_rev[0], a.ev_rev[1]})
+    if (e) (void)hipEventDestroy(e);
+  if (a.rec_stream) (void)hipStreamDestroy(a.rec_stream);
   a = AdjointBuffers{};
 }
 
@@ -74,29 +81,40 @@ void Engine::adjoint_buffers(int K, int C) {
     a.first_bad = dalloc<int>(R);
     a.sexp = dalloc<int>(2 * (size_t)R);
     a.mx = dalloc<unsigned>(2 * (size_t)R);
+    HIP_OK(hipStreamCreateWithFlags(&a.rec_stream, hipStreamNonBlocking));
+    for (hipEvent_t* e : {&a.ev_fwd, &a.ev_rec[0], &a.ev_rec[1], &a.ev_rev[0], &a.ev_rev[1]})
+      HIP_OK(hipEventCreateWithFlags(e, hipEventDisableTiming));
   }
   if (C > a.C) {
     dfree(a.ck_xycs); dfree(a.ck_th); dfree(a.ck_vid);
     a.ck_xycs = dalloc<float4>((size_t)C * N); a.ck_th = dalloc<float>((size_t)C * N); a.ck_vid = dalloc<int>((size_t)C * N);
     a.C = C;
   }
-  if (K > a.K) {
-    dfree(a.tp_xycs); dfree(a.tp_th); dfree(a.tp_vid); dfree(a.tp_fs); dfree(a.tp_q);
-    a.tp_xycs = dalloc<float4>((size_t)(K + 1) * N);
-    a.tp_th = dalloc<float>((size_t)(K + 1) * N);
-    a.tp_vid = dalloc<int>((size_t)(K + 1) * N);
-    a.tp_fs = dalloc<float4>((size_t)K * N);
-    a.tp_q = dalloc<int>((size_t)K * N);
-    a.K = K;
-  }
-  // the buckets of every state of a segment, kept for its reverse steps, within about 16 GB
-  // (otherwise each reverse step re-buckets its state)
-  const size_t want = (size_t)K + 2;
-  if (a.sbs.size() < want && want * bucket_set_bytes() <= (size_t(16) << 30)) {
-    while (a.sbs.size() < want) {
-      a.sbs.emplace_back();
-      alloc_bucket_set(a.sbs.back());
+  // two tapes when the bucket sets of both fit in about 16 GB (the recompute of one segment then
+  // overlaps the reverse pass of the next); else one tape, with its states' bucket sets when
+  // they fit (otherwise each reverse step re-buckets its state)
+  const size_t sets = (size_t)K + 2, budget = size_t(16) << 30;
+  const int ntape = 2 * sets * bucket_set_bytes() <= budget ? 2 : 1;
+  if (K > a.K || ntape != a.ntape) {
+    for (Tape& t : a.tp) {
+      dfree(t.xycs); dfree(t.th); dfree(t.vid); dfree(t.fs); dfree(t.q);
+      for (BucketSet& b : t.sbs) free_bucket_set(b);
+      t.sbs.clear();
     }
+    for (int k = 0; k < ntape; ++k) {
+      Tape& t = a.tp[k];
+      t.xycs = dalloc<float4>((size_t)(K + 1) * N);
+      t.th = dalloc<float>((size_t)(K + 1) * N);
+      t.vid = dalloc<int>((size_t)(K + 1) * N);
+      t.fs = dalloc<float4>((size_t)K * N);
+      t.q = dalloc<int>((size_t)K * N);
+      if (sets * bucket_set_bytes() <= budget) {
+        t.sbs.resize(sets);
+        for (BucketSet& b : t.sbs) alloc_bucket_set(b);
+      }
+    }
+    a.K = K;
+    a.ntape = ntape;
   }
   if (a.partial_n < n_logical) {
     dfree(a.partial);
@@ -139,26 +157,34 @@ int32_t Engine::adjoint(int T, const float* nz, size_t stride, int K, const tiss
   HIP_OK(hipMemsetAsync(a.acc, 0, kAdjAcc * N * sizeof(double), stream));
   HIP_OK(hipMemsetAsync(a.sexp, 0, 2 * R * sizeof(int), stream));
   HIP_OK(hipMemsetAsync(a.mx, 0, 2 * R * sizeof(unsigned), stream));
-  // lr[cur], lt[cur]: the adjoint of the current state; slot cur of sexp and mx: its exponent and max
-  int cur = 0;
-  for (int c = C - 1; c >= 0; --c) {
+  // recomputes run on rs: with two tapes a second stream, so that the recompute of segment c - 1
+  // overlaps the reverse pass of segment c (tape c % 2 is handed over by events); with one tape
+  // everything runs in order on the engine's stream
+  const bool overlap = a.ntape == 2;
+  hipStream_t rs = overlap ? a.rec_stream : stream;
+  if (overlap) {
+    HIP_OK(hipEventRecord(a.ev_fwd, stream));  // the checkpoints are written
+    HIP_OK(hipStreamWaitEvent(rs, a.ev_fwd, 0));
+  }
+  auto segment_set = [&](Tape& tp, int kc, int q) -> BucketSet& {
+    return tp.sbs.size() >= (size_t)kc + 2 ? tp.sbs[q] : bs[q % 3];
+  };
+  // recompute segment c from its checkpoint into tape c % ntape, recording the tape
+  auto recompute = [&](int c) {
     const int t0 = c * K, kc = std::min(K, T - t0);
-    // recompute the segment from its checkpoint, recording the tape
-    HIP_OK(hipMemcpyAsync(a.tp_xycs, a.ck_xycs + c * N, N * sizeof(float4), hipMemcpyDeviceToDevice, stream));
-    HIP_OK(hipMemcpyAsync(a.tp_th, a.ck_th + c * N, N * sizeof(float), hipMemcpyDeviceToDevice, stream));
-    HIP_OK(hipMemcpyAsync(a.tp_vid, a.ck_vid + c * N, N * sizeof(int), hipMemcpyDeviceToDevice, stream));
-    // bucket sets of the segment: the states' own (set q holds state q's buckets) when they fit,
-    // else the three rotating sets, with each reverse step re-bucketing its state
-    const bool keep = a.sbs.size() >= (size_t)kc + 2;
-    auto set = [&](int q) -> BucketSet& { return keep ? a.sbs[q] : bs[q % 3]; };
-    clear_bucket_set(set(1));
-    clear_bucket_set(set(2));
-    bucket_state_into(a.tp_vid, set(0));
-    HIP_OK(launch_flags_begin(R, d_halt, a.rows, a.first_bad, stream));
+    Tape& tp = a.tp[c % a.ntape];
+    if (overlap && c + 2 <= C - 1) HIP_OK(hipStreamWaitEvent(rs, a.ev_rev[c % 2], 0));  // segment c + 2 is done with it
+    HIP_OK(hipMemcpyAsync(tp.xycs, a.ck_xycs + c * N, N * sizeof(float4), hipMemcpyDeviceToDevice, rs));
+    HIP_OK(hipMemcpyAsync(tp.th, a.ck_th + c * N, N * sizeof(float), hipMemcpyDeviceToDevice, rs));
+    HIP_OK(hipMemcpyAsync(tp.vid, a.ck_vid + c * N, N * sizeof(int), hipMemcpyDeviceToDevice, rs));
+    clear_bucket_set(segment_set(tp, kc, 1), rs);
+    clear_bucket_set(segment_set(tp, kc, 2), rs);
+    bucket_state_into(tp.vid, segment_set(tp, kc, 0), rs);
+    HIP_OK(launch_flags_begin(R, d_halt, a.rows, a.first_bad, rs));
     for (int q = 0; q < kc; ++q) {
       TiledIO io{};
-      io.xycs_in = a.tp_xycs + q * N; io.th_in = a.tp_th + q * N; io.vid_in = a.tp_vid + q * N;
-      io.xycs_out = a.tp_xycs + (q + 1) * N; io.th_out = a.tp_th + (q + 1) * N; io.vid_out = a.tp_vid + (q + 1) * N;
+      io.xycs_in = tp.xycs + q * N; io.th_in = tp.th + q * N; io.vid_in = tp.vid + q * N;
+      io.xycs_out = tp.xycs + (q + 1) * N; io.th_out = tp.th + (q + 1) * N; io.vid_out = tp.vid + (q + 1) * N;
       io.vx = a.vx; io.vy = a.vy; io.r3x = a.r3x; io.r3y = a.r3y; io.r3z = a.r3z; io.face = a.face; io.nbr = a.nbr;
       io.noise = nz ? nz + (size_t)(t0 + q) * stride : nullptr;
       io.partial = a.partial;
@@ -167,23 +193,40 @@ int32_t Engine::adjoint(int T, const float* nz, size_t stride, int K, const tiss
       io.halt_clr = a.rows + (size_t)((q + 2) % 3) * R;
       io.first_bad = a.first_bad;
       io.step = q;
-      io.tape_fs = a.tp_fs + q * N;
-      io.tape_q = a.tp_q + q * N;
-      HIP_OK(launch_step_tiled(s.lanes, s.block, true, md, d_reps, d_tiles, n_phys, set(q), set(q + 1), set(q + 2), io,
-                               stream));
-      HIP_OK(launch_bucket_finalize(s.block, d_reps, d_tiles, n_phys, md.nv, R == 1 ? ntot : 0, set(q + 1), stream));
+      io.tape_fs = tp.fs + q * N;
+      io.tape_q = tp.q + q * N;
+      HIP_OK(launch_step_tiled(s.lanes, s.block, true, md, d_reps, d_tiles, n_phys, segment_set(tp, kc, q),
+                               segment_set(tp, kc, q + 1), segment_set(tp, kc, q + 2), io, rs));
+      HIP_OK(launch_bucket_finalize(s.block, d_reps, d_tiles, n_phys, md.nv, R == 1 ? ntot : 0,
+                                    segment_set(tp, kc, q + 1), rs));
     }
-    // backwards through the segment
+    if (overlap) HIP_OK(hipEventRecord(a.ev_rec[c % 2], rs));
+  };
+  // lr[cur], lt[cur]: the adjoint of the current state; slot cur of sexp and mx: its exponent and max
+  int cur = 0;
+  // backwards through segment c
+  auto reverse = [&](int c) {
+    const int t0 = c * K, kc = std::min(K, T - t0);
+    Tape& tp = a.tp[c % a.ntape];
+    const bool keep = tp.sbs.size() >= (size_t)kc + 2;
+    if (overlap) HIP_OK(hipStreamWaitEvent(stream, a.ev_rec[c % 2], 0));
     for (int q = kc - 1; q >= 0; --q) {
-      if (!keep) bucket_state_into(a.tp_vid + q * N, bs[0]);
+      if (!keep) bucket_state_into(tp.vid + q * N, bs[0], stream);
       HIP_OK(launch_adj_local(d_reps, d_rid, ntot, L, d_halt, a.sexp + cur * R, a.sexp + (1 - cur) * R, a.mx + cur * R,
-                              a.mx + (1 - cur) * R, a.tp_xycs + q * N, a.tp_th + q * N, a.tp_fs + q * N,
-                              a.tp_q + q * N, a.lr[cur], a.lt[cur], a.gF, a.gSC, a.part, a.acc, stream));
-      HIP_OK(launch_adj_gather(s.lanes, md, d_reps, d_rid, ntot, keep ? a.sbs[q] : bs[0], d_halt, a.tp_xycs + q * N,
-                               a.tp_vid + q * N, a.gF, a.gSC, a.part, a.lr[1 - cur], a.lt[1 - cur],
+                              a.mx + (1 - cur) * R, tp.xycs + q * N, tp.th + q * N, tp.fs + q * N, tp.q + q * N,
+                              a.lr[cur], a.lt[cur], a.gF, a.gSC, a.part, a.acc, stream));
+      HIP_OK(launch_adj_gather(s.lanes, md, d_reps, d_rid, ntot, keep ? tp.sbs[q] : bs[0], d_halt, tp.xycs + q * N,
+                               tp.vid + q * N, a.gF, a.gSC, a.part, a.lr[1 - cur], a.lt[1 - cur],
                                a.mx + (1 - cur) * R, stream));
       cur = 1 - cur;
     }
+    if (overlap) HIP_OK(hipEventRecord(a.ev_rev[c % 2], stream));
+  };
+  recompute(C - 1);
+  for (int c = C - 1; c >= 0; --c) {
+    if (overlap && c >= 1) recompute(c - 1);
+    reverse(c);
+    if (!overlap && c >= 1) recompute(c - 1);
   }
   HIP_OK(launch_adj_reduce(R, d_reps, ntot, a.acc, a.out_red, stream));
   HIP_OK(hipEventRecord(ev1, stream));
