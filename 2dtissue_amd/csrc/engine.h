@@ -125,32 +125,36 @@ struct Engine {
   std::map<std::tuple<int, const float*, size_t, std::tuple<int, int, int, int, int>, int, int>, hipGraphExec_t> graphs;
 
   // one segment's tape: states [K+1][ntot], force / alignment sums and seam-wrap turns [K][ntot],
-  // and the bucket sets of its states when they fit (else empty: each reverse step re-buckets)
+  // the bucket sets of its states when they fit (else empty: each reverse step re-buckets), and
+  // the scratch its recompute writes (flag rows, order partials, per-step outputs)
   struct Tape {
     float4* xycs = nullptr; float* th = nullptr; int* vid = nullptr;
     float4* fs = nullptr; int* q = nullptr;
     std::vector<BucketSet> sbs;
-  };
-  // adjoint buffers (grown on demand): checkpoints [C][ntot], tapes, adjoint state, factors.
-  // With two tapes the recompute of a segment (on rec_stream) runs while the reverse pass of the
-  // following segment (on the engine's stream) runs; events order the tape hand-overs.
-  struct AdjointBuffers {
-    int K = 0, C = 0;
-    float4* ck_xycs = nullptr; float* ck_th = nullptr; int* ck_vid = nullptr;
-    Tape tp[2];
-    int ntape = 0;
-    hipStream_t rec_stream = nullptr;
-    hipEvent_t ev_fwd = nullptr, ev_rec[2] = {nullptr, nullptr}, ev_rev[2] = {nullptr, nullptr};
-    float2* lr[2] = {nullptr, nullptr}; float* lt[2] = {nullptr, nullptr};
-    float2* gF = nullptr; float2* gSC = nullptr; float4* part = nullptr;
-    double* acc = nullptr; double* out_seed = nullptr; double* out_red = nullptr;
     float *vx = nullptr, *vy = nullptr, *r3x = nullptr, *r3y = nullptr, *r3z = nullptr;
     int *face = nullptr, *nbr = nullptr;
     unsigned* rows = nullptr; int* first_bad = nullptr;
+    OrderPartial* partial = nullptr; int partial_n = 0;
+  };
+  static constexpr int kMaxTapes = 3;
+  // adjoint buffers (grown on demand): checkpoints [C][ntot], tapes, adjoint state, factors.
+  // With ntape > 1 tapes, segment c is recomputed into tape c % ntape on recompute stream
+  // c % (ntape - 1) while the reverse passes of later segments run on the engine's stream;
+  // events hand the tapes over.
+  struct AdjointBuffers {
+    int K = 0, C = 0;
+    float4* ck_xycs = nullptr; float* ck_th = nullptr; int* ck_vid = nullptr;
+    Tape tp[kMaxTapes];
+    int ntape = 0;
+    hipStream_t rec_stream[kMaxTapes - 1] = {};
+    hipEvent_t ev_fwd = nullptr, ev_rec[kMaxTapes] = {}, ev_rev[kMaxTapes] = {};
+    float2* lr[2] = {nullptr, nullptr}; float* lt[2] = {nullptr, nullptr};
+    float2* gF = nullptr; float2* gSC = nullptr; float4* part = nullptr;
+    double* acc = nullptr; double* out_seed = nullptr; double* out_red = nullptr;
     int* sexp = nullptr;  // [2][R]: the adjoint state is stored scaled by 2^-sexp (slots alternate by step)
     unsigned* mx = nullptr;  // [2][R]: max |lambda| of the last gather's output (float bits)
-    OrderPartial* partial = nullptr; int partial_n = 0;
   } adj;
+  void free_tape(Tape& t);
   void adjoint_buffers(int K, int C);
   void adjoint_release();
 

@@ -7,8 +7,8 @@
 // buckets of every state of the segment (when they fit in about 16 GB; otherwise each reverse
 // step rebuilds its state's buckets from the recorded vertex ids); then, step by step backwards,
 // the two reverse kernels (per-particle factors, then the transposed neighbour gather over the
-// step's buckets) map the adjoint of the new state to the adjoint of the old one. With two tapes
-// the recompute of the previous segment runs on a second stream during that reverse pass.
+// step's buckets) map the adjoint of the new state to the adjoint of the old one. With several
+// tapes the recomputes of the next segments run on their own streams during that reverse pass.
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -47,21 +47,31 @@ LossDev make_loss(const tissue_loss& l, int n_steps) {
 
 }  // namespace
 
+void Engine::free_tape(Tape& t) {
+  dfree(t.xycs); dfree(t.th); dfree(t.vid); dfree(t.fs); dfree(t.q);
+  for (BucketSet& b : t.sbs) free_bucket_set(b);
+  t.sbs.clear();
+  dfree(t.vx); dfree(t.vy); dfree(t.r3x); dfree(t.r3y); dfree(t.r3z); dfree(t.face); dfree(t.nbr);
+  dfree(t.rows); dfree(t.first_bad); dfree(t.partial);
+  t.partial_n = 0;
+}
+
 void Engine::adjoint_release() {
   AdjointBuffers& a = adj;
-  if (a.rec_stream) (void)hipStreamSynchronize(a.rec_stream);
+  for (hipStream_t st : a.rec_stream)
+    if (st) (void)hipStreamSynchronize(st);
   dfree(a.ck_xycs); dfree(a.ck_th); dfree(a.ck_vid);
-  for (Tape& t : a.tp) {
-    dfree(t.xycs); dfree(t.th); dfree(t.vid); dfree(t.fs); dfree(t.q);
-    for (BucketSet& b : t.sbs) free_bucket_set(b);
-  }
+  for (Tape& t : a.tp) free_tape(t);
   for (int k = 0; k < 2; ++k) { dfree(a.lr[k]); dfree(a.lt[k]); }
   dfree(a.gF); dfree(a.gSC); dfree(a.part); dfree(a.acc); dfree(a.out_seed); dfree(a.out_red);
-  dfree(a.vx); dfree(a.vy); dfree(a.r3x); dfree(a.r3y); dfree(a.r3z); dfree(a.face); dfree(a.nbr);
-  dfree(a.rows); dfree(a.first_bad); dfree(a.partial); dfree(a.sexp); dfree(a.mx);
-  for (hipEvent_t e : {a.ev_fwd, a.ev_rec[0], a.ev_rec[1], a.ev_rev[0], a.ev_rev[1]})
-    if (e) (void)hipEventDestroy(e);
-  if (a.rec_stream) (void)hipStreamDestroy(a.rec_stream);
+  dfree(a.sexp); dfree(a.mx);
+  if (a.ev_fwd) (void)hipEventDestroy(a.ev_fwd);
+  for (int k = 0; k < kMaxTapes; ++k) {
+    if (a.ev_rec[k]) (void)hipEventDestroy(a.ev_rec[k]);
+    if (a.ev_rev[k]) (void)hipEventDestroy(a.ev_rev[k]);
+  }
+  for (hipStream_t st : a.rec_stream)
+    if (st) (void)hipStreamDestroy(st);
   a = AdjointBuffers{};
 }
 
@@ -74,33 +84,29 @@ void Engine::adjoint_buffers(int K, int C) {
     a.acc = dalloc<double>(kAdjAcc * N);
     a.out_seed = dalloc<double>(2 * (size_t)R);
     a.out_red = dalloc<double>(kAdjAcc * (size_t)R);
-    a.vx = dalloc<float>(N); a.vy = dalloc<float>(N);
-    a.r3x = dalloc<float>(N); a.r3y = dalloc<float>(N); a.r3z = dalloc<float>(N);
-    a.face = dalloc<int>(N); a.nbr = dalloc<int>(N);
-    a.rows = dalloc<unsigned>(3 * (size_t)R);
-    a.first_bad = dalloc<int>(R);
     a.sexp = dalloc<int>(2 * (size_t)R);
     a.mx = dalloc<unsigned>(2 * (size_t)R);
-    HIP_OK(hipStreamCreateWithFlags(&a.rec_stream, hipStreamNonBlocking));
-    for (hipEvent_t* e : {&a.ev_fwd, &a.ev_rec[0], &a.ev_rec[1], &a.ev_rev[0], &a.ev_rev[1]})
-      HIP_OK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    for (hipStream_t& st : a.rec_stream) HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    HIP_OK(hipEventCreateWithFlags(&a.ev_fwd, hipEventDisableTiming));
+    for (int k = 0; k < kMaxTapes; ++k) {
+      HIP_OK(hipEventCreateWithFlags(&a.ev_rec[k], hipEventDisableTiming));
+      HIP_OK(hipEventCreateWithFlags(&a.ev_rev[k], hipEventDisableTiming));
+    }
   }
   if (C > a.C) {
     dfree(a.ck_xycs); dfree(a.ck_th); dfree(a.ck_vid);
     a.ck_xycs = dalloc<float4>((size_t)C * N); a.ck_th = dalloc<float>((size_t)C * N); a.ck_vid = dalloc<int>((size_t)C * N);
     a.C = C;
   }
-  // two tapes when the bucket sets of both fit in about 16 GB (the recompute of one segment then
-  // overlaps the reverse pass of the next); else one tape, with its states' bucket sets when
-  // they fit (otherwise each reverse step re-buckets its state)
+  // as many tapes (up to three) as have their bucket sets within about 16 GB together: the
+  // recomputes of two segments then run side by side while the reverse pass of a third runs;
+  // else one tape, with its states' bucket sets when they fit (otherwise each reverse step
+  // re-buckets its state), and everything in order on the engine's stream
   const size_t sets = (size_t)K + 2, budget = size_t(16) << 30;
-  const int ntape = 2 * sets * bucket_set_bytes() <= budget ? 2 : 1;
+  int ntape = 1;
+  while (ntape < kMaxTapes && (size_t)(ntape + 1) * sets * bucket_set_bytes() <= budget) ++ntape;
   if (K > a.K || ntape != a.ntape) {
-    for (Tape& t : a.tp) {
-      dfree(t.xycs); dfree(t.th); dfree(t.vid); dfree(t.fs); dfree(t.q);
-      for (BucketSet& b : t.sbs) free_bucket_set(b);
-      t.sbs.clear();
-    }
+    for (Tape& t : a.tp) free_tape(t);
     for (int k = 0; k < ntape; ++k) {
       Tape& t = a.tp[k];
       t.xycs = dalloc<float4>((size_t)(K + 1) * N);
@@ -112,14 +118,22 @@ void Engine::adjoint_buffers(int K, int C) {
         t.sbs.resize(sets);
         for (BucketSet& b : t.sbs) alloc_bucket_set(b);
       }
+      t.vx = dalloc<float>(N); t.vy = dalloc<float>(N);
+      t.r3x = dalloc<float>(N); t.r3y = dalloc<float>(N); t.r3z = dalloc<float>(N);
+      t.face = dalloc<int>(N); t.nbr = dalloc<int>(N);
+      t.rows = dalloc<unsigned>(3 * (size_t)R);
+      t.first_bad = dalloc<int>(R);
     }
     a.K = K;
     a.ntape = ntape;
   }
-  if (a.partial_n < n_logical) {
-    dfree(a.partial);
-    a.partial = dalloc<OrderPartial>(n_logical);
-    a.partial_n = n_logical;
+  for (int k = 0; k < ntape; ++k) {
+    Tape& t = a.tp[k];
+    if (t.partial_n < n_logical) {
+      dfree(t.partial);
+      t.partial = dalloc<OrderPartial>(n_logical);
+      t.partial_n = n_logical;
+    }
   }
 }
 
@@ -157,41 +171,44 @@ int32_t Engine::adjoint(int T, const float* nz, size_t stride, int K, const tiss
   HIP_OK(hipMemsetAsync(a.acc, 0, kAdjAcc * N * sizeof(double), stream));
   HIP_OK(hipMemsetAsync(a.sexp, 0, 2 * R * sizeof(int), stream));
   HIP_OK(hipMemsetAsync(a.mx, 0, 2 * R * sizeof(unsigned), stream));
-  // recomputes run on rs: with two tapes a second stream, so that the recompute of segment c - 1
-  // overlaps the reverse pass of segment c (tape c % 2 is handed over by events); with one tape
+  // recomputes: with nt > 1 tapes, segment c goes to tape c % nt on recompute stream c % (nt - 1),
+  // nt - 1 segments ahead of the reverse passes (tapes are handed over by events); with one tape
   // everything runs in order on the engine's stream
-  const bool overlap = a.ntape == 2;
-  hipStream_t rs = overlap ? a.rec_stream : stream;
-  if (overlap) {
-    HIP_OK(hipEventRecord(a.ev_fwd, stream));  // the checkpoints are written
-    HIP_OK(hipStreamWaitEvent(rs, a.ev_fwd, 0));
-  }
+  const int nt = a.ntape, ns = nt - 1;
+  const bool overlap = nt > 1;
+  if (overlap) HIP_OK(hipEventRecord(a.ev_fwd, stream));  // the checkpoints are written
+  auto rs_of = [&](int c) { return overlap ? a.rec_stream[c % ns] : stream; };
   auto segment_set = [&](Tape& tp, int kc, int q) -> BucketSet& {
     return tp.sbs.size() >= (size_t)kc + 2 ? tp.sbs[q] : bs[q % 3];
   };
-  // recompute segment c from its checkpoint into tape c % ntape, recording the tape
+  // recompute segment c from its checkpoint into tape c % nt, recording the tape
   auto recompute = [&](int c) {
     const int t0 = c * K, kc = std::min(K, T - t0);
-    Tape& tp = a.tp[c % a.ntape];
-    if (overlap && c + 2 <= C - 1) HIP_OK(hipStreamWaitEvent(rs, a.ev_rev[c % 2], 0));  // segment c + 2 is done with it
+    Tape& tp = a.tp[c % nt];
+    hipStream_t rs = rs_of(c);
+    if (overlap) {
+      HIP_OK(hipStreamWaitEvent(rs, a.ev_fwd, 0));
+      if (c + nt <= C - 1) HIP_OK(hipStreamWaitEvent(rs, a.ev_rev[c % nt], 0));  // segment c + nt is done with it
+    }
     HIP_OK(hipMemcpyAsync(tp.xycs, a.ck_xycs + c * N, N * sizeof(float4), hipMemcpyDeviceToDevice, rs));
     HIP_OK(hipMemcpyAsync(tp.th, a.ck_th + c * N, N * sizeof(float), hipMemcpyDeviceToDevice, rs));
     HIP_OK(hipMemcpyAsync(tp.vid, a.ck_vid + c * N, N * sizeof(int), hipMemcpyDeviceToDevice, rs));
     clear_bucket_set(segment_set(tp, kc, 1), rs);
     clear_bucket_set(segment_set(tp, kc, 2), rs);
     bucket_state_into(tp.vid, segment_set(tp, kc, 0), rs);
-    HIP_OK(launch_flags_begin(R, d_halt, a.rows, a.first_bad, rs));
+    HIP_OK(launch_flags_begin(R, d_halt, tp.rows, tp.first_bad, rs));
     for (int q = 0; q < kc; ++q) {
       TiledIO io{};
       io.xycs_in = tp.xycs + q * N; io.th_in = tp.th + q * N; io.vid_in = tp.vid + q * N;
       io.xycs_out = tp.xycs + (q + 1) * N; io.th_out = tp.th + (q + 1) * N; io.vid_out = tp.vid + (q + 1) * N;
-      io.vx = a.vx; io.vy = a.vy; io.r3x = a.r3x; io.r3y = a.r3y; io.r3z = a.r3z; io.face = a.face; io.nbr = a.nbr;
+      io.vx = tp.vx; io.vy = tp.vy; io.r3x = tp.r3x; io.r3y = tp.r3y; io.r3z = tp.r3z; io.face = tp.face;
+      io.nbr = tp.nbr;
       io.noise = nz ? nz + (size_t)(t0 + q) * stride : nullptr;
-      io.partial = a.partial;
-      io.halt_in = a.rows + (size_t)(q % 3) * R;
-      io.halt_out = a.rows + (size_t)((q + 1) % 3) * R;
-      io.halt_clr = a.rows + (size_t)((q + 2) % 3) * R;
-      io.first_bad = a.first_bad;
+      io.partial = tp.partial;
+      io.halt_in = tp.rows + (size_t)(q % 3) * R;
+      io.halt_out = tp.rows + (size_t)((q + 1) % 3) * R;
+      io.halt_clr = tp.rows + (size_t)((q + 2) % 3) * R;
+      io.first_bad = tp.first_bad;
       io.step = q;
       io.tape_fs = tp.fs + q * N;
       io.tape_q = tp.q + q * N;
@@ -200,16 +217,16 @@ int32_t Engine::adjoint(int T, const float* nz, size_t stride, int K, const tiss
       HIP_OK(launch_bucket_finalize(s.block, d_reps, d_tiles, n_phys, md.nv, R == 1 ? ntot : 0,
                                     segment_set(tp, kc, q + 1), rs));
     }
-    if (overlap) HIP_OK(hipEventRecord(a.ev_rec[c % 2], rs));
+    if (overlap) HIP_OK(hipEventRecord(a.ev_rec[c % nt], rs));
   };
   // lr[cur], lt[cur]: the adjoint of the current state; slot cur of sexp and mx: its exponent and max
   int cur = 0;
   // backwards through segment c
   auto reverse = [&](int c) {
     const int t0 = c * K, kc = std::min(K, T - t0);
-    Tape& tp = a.tp[c % a.ntape];
+    Tape& tp = a.tp[c % nt];
     const bool keep = tp.sbs.size() >= (size_t)kc + 2;
-    if (overlap) HIP_OK(hipStreamWaitEvent(stream, a.ev_rec[c % 2], 0));
+    if (overlap) HIP_OK(hipStreamWaitEvent(stream, a.ev_rec[c % nt], 0));
     for (int q = kc - 1; q >= 0; --q) {
       if (!keep) bucket_state_into(tp.vid + q * N, bs[0], stream);
       HIP_OK(launch_adj_local(d_reps, d_rid, ntot, L, d_halt, a.sexp + cur * R, a.sexp + (1 - cur) * R, a.mx + cur * R,
@@ -220,13 +237,20 @@ int32_t Engine::adjoint(int T, const float* nz, size_t stride, int K, const tiss
                                a.mx + (1 - cur) * R, stream));
       cur = 1 - cur;
     }
-    if (overlap) HIP_OK(hipEventRecord(a.ev_rev[c % 2], stream));
+    if (overlap) HIP_OK(hipEventRecord(a.ev_rev[c % nt], stream));
   };
-  recompute(C - 1);
-  for (int c = C - 1; c >= 0; --c) {
-    if (overlap && c >= 1) recompute(c - 1);
-    reverse(c);
-    if (!overlap && c >= 1) recompute(c - 1);
+  if (overlap) {
+    for (int k = 0; k < ns && C - 1 - k >= 0; ++k) recompute(C - 1 - k);
+    for (int c = C - 1; c >= 0; --c) {
+      reverse(c);
+      if (c - ns >= 0) recompute(c - ns);
+    }
+  } else {
+    recompute(C - 1);
+    for (int c = C - 1; c >= 0; --c) {
+      reverse(c);
+      if (c >= 1) recompute(c - 1);
+    }
   }
   HIP_OK(launch_adj_reduce(R, d_reps, ntot, a.acc, a.out_red, stream));
   HIP_OK(hipEventRecord(ev1, stream));
