@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <cstdlib>
 #include <cmath>
 
 #include "engine.h"
@@ -102,10 +103,18 @@ void Engine::adjoint_buffers(int K, int C) {
   // recomputes of two segments then run side by side while the reverse pass of a third runs;
   // else one tape, with its states' bucket sets when they fit (otherwise each reverse step
   // re-buckets its state), and everything in order on the engine's stream
-  const size_t sets = (size_t)K + 2, budget = size_t(16) << 30;
+  // (the budget is TISSUE_ADJOINT_BUCKET_KB kilobytes when that is set: a smaller one selects the
+  // one-tape and re-bucketing forms, which give the same gradients)
+  const size_t sets = (size_t)K + 2;
+  size_t budget = size_t(16) << 30;
+  if (const char* e = std::getenv("TISSUE_ADJOINT_BUCKET_KB")) {
+    const long long kb = std::atoll(e);
+    if (kb >= 0) budget = (size_t)kb << 10;
+  }
   int ntape = 1;
   while (ntape < kMaxTapes && (size_t)(ntape + 1) * sets * bucket_set_bytes() <= budget) ++ntape;
-  if (K > a.K || ntape != a.ntape) {
+  const bool with_sets = sets * bucket_set_bytes() <= budget;
+  if (K > a.K || ntape != a.ntape || with_sets != !a.tp[0].sbs.empty()) {
     for (Tape& t : a.tp) free_tape(t);
     for (int k = 0; k < ntape; ++k) {
       Tape& t = a.tp[k];
@@ -114,7 +123,7 @@ void Engine::adjoint_buffers(int K, int C) {
       t.vid = dalloc<int>((size_t)(K + 1) * N);
       t.fs = dalloc<float4>((size_t)K * N);
       t.q = dalloc<int>((size_t)K * N);
-      if (sets * bucket_set_bytes() <= budget) {
+      if (with_sets) {
         t.sbs.resize(sets);
         for (BucketSet& b : t.sbs) alloc_bucket_set(b);
       }

@@ -100,6 +100,36 @@ def test_adjoint_checkpoint_interval_is_exact(T):
         assert np.array_equal(dth, res[0][1]) and np.array_equal(dx, res[0][2]) and np.array_equal(dy, res[0][3])
 
 
+def test_adjoint_tape_forms_are_exact(T, monkeypatch):
+    """The reverse pass in its three forms gives bitwise the same gradients: several tapes whose
+    recomputes overlap the reverse passes (the default), one tape with its states' bucket sets,
+    and one tape that re-buckets each state (memory budgets set by TISSUE_ADJOINT_BUCKET_KB)."""
+    cfg = P.C1
+    m, _ = P.mesh_and_tables(cfg["freq"], cfg["sigma"])
+    p = dict(P.PHYS, sigma=cfg["sigma"])
+    Lc = T.loss(0.8, 1.0, 0.1, T.target_field(4))
+    noise = T.gen_noise(2, 0.1, 15, cfg["n"])
+    K = 4
+    set_bytes = m.info()["nv"] * 36 + cfg["n"] * 16 + 12
+    one_tape_kb = int(1.5 * (K + 2) * set_bytes) // 1024
+    res = []
+    for budget in (None, one_tape_kb, 0):
+        if budget is None:
+            monkeypatch.delenv("TISSUE_ADJOINT_BUCKET_KB", raising=False)
+        else:
+            monkeypatch.setenv("TISSUE_ADJOINT_BUCKET_KB", str(budget))
+        s = make(T, m, p, cfg["n"], 0)
+        s.stage_noise(noise)
+        code, g, dth, dx, dy = s.adjoint(15, Lc, 0, K)
+        assert code == T.OK
+        res.append((g, dth, dx, dy))
+        s.close()
+    for g, dth, dx, dy in res[1:]:
+        for key in ("d_tau", "d_mu", "d_v0", "d_k", "loss"):
+            assert g[key] == res[0][0][key], key
+        assert np.array_equal(dth, res[0][1]) and np.array_equal(dx, res[0][2]) and np.array_equal(dy, res[0][3])
+
+
 def test_batch_adjoint_equals_solo(T):
     sigma = 0.05
     m, _ = P.mesh_and_tables(32, sigma)
