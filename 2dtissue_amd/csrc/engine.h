@@ -147,7 +147,7 @@ struct Engine {
     Tape tp[kMaxTapes];
     int ntape = 0;
     hipStream_t rec_stream[kMaxTapes - 1] = {};
-    hipEvent_t ev_fwd = nullptr, ev_rec[kMaxTapes] = {}, ev_rev[kMaxTapes] = {};
+    hipEvent_t ev_fwd = nullptr, ev_ck[kMaxTapes] = {}, ev_rec[kMaxTapes] = {}, ev_rev[kMaxTapes] = {};
     float2* lr[2] = {nullptr, nullptr}; float* lt[2] = {nullptr, nullptr};
     float2* gF = nullptr; float2* gSC = nullptr; float4* part = nullptr;
     double* acc = nullptr; double* out_seed = nullptr; double* out_red = nullptr;

@@ -68,6 +68,7 @@ void Engine::adjoint_release() {
   dfree(a.sexp); dfree(a.mx);
   if (a.ev_fwd) (void)hipEventDestroy(a.ev_fwd);
   for (int k = 0; k < kMaxTapes; ++k) {
+    if (a.ev_ck[k]) (void)hipEventDestroy(a.ev_ck[k]);
     if (a.ev_rec[k]) (void)hipEventDestroy(a.ev_rec[k]);
     if (a.ev_rev[k]) (void)hipEventDestroy(a.ev_rev[k]);
   }
@@ -90,6 +91,7 @@ void Engine::adjoint_buffers(int K, int C) {
     for (hipStream_t& st : a.rec_stream) HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     HIP_OK(hipEventCreateWithFlags(&a.ev_fwd, hipEventDisableTiming));
     for (int k = 0; k < kMaxTapes; ++k) {
+      HIP_OK(hipEventCreateWithFlags(&a.ev_ck[k], hipEventDisableTiming));
       HIP_OK(hipEventCreateWithFlags(&a.ev_rec[k], hipEventDisableTiming));
       HIP_OK(hipEventCreateWithFlags(&a.ev_rev[k], hipEventDisableTiming));
     }
@@ -161,42 +163,25 @@ int32_t Engine::adjoint(int T, const float* nz, size_t stride, int K, const tiss
   AdjointBuffers& a = adj;
   const size_t N = ntot;
 
-  // forward, with a checkpoint at the start of every segment
-  double fwd_ms = 0.0;
-  int32_t status = TISSUE_OK;
-  for (int c = 0; c < C; ++c) {
-    const int t0 = c * K, kc = std::min(K, T - t0);
-    HIP_OK(hipMemcpyAsync(a.ck_xycs + c * N, xycs[sphase], N * sizeof(float4), hipMemcpyDeviceToDevice, stream));
-    HIP_OK(hipMemcpyAsync(a.ck_th + c * N, th[sphase], N * sizeof(float), hipMemcpyDeviceToDevice, stream));
-    HIP_OK(hipMemcpyAsync(a.ck_vid + c * N, vid[sphase], N * sizeof(int), hipMemcpyDeviceToDevice, stream));
-    const int32_t st = run_shape(s, kc, nz, stride, t0);
-    fwd_ms += last_ms;
-    if (st != TISSUE_OK && status == TISSUE_OK) status = st;
-  }
-
-  // reverse
-  HIP_OK(hipEventRecord(ev0, stream));
-  HIP_OK(launch_adj_seed(R, d_reps, xycs[sphase], L, d_halt, a.lr[0], a.lt[0], a.out_seed, stream));
-  HIP_OK(hipMemsetAsync(a.acc, 0, kAdjAcc * N * sizeof(double), stream));
-  HIP_OK(hipMemsetAsync(a.sexp, 0, 2 * R * sizeof(int), stream));
-  HIP_OK(hipMemsetAsync(a.mx, 0, 2 * R * sizeof(unsigned), stream));
   // recomputes: with nt > 1 tapes, segment c goes to tape c % nt on recompute stream c % (nt - 1),
   // nt - 1 segments ahead of the reverse passes (tapes are handed over by events); with one tape
   // everything runs in order on the engine's stream
+  // (the last nt - 1 segments are recomputed as soon as their checkpoints are written, while the
+  // forward still runs)
   const int nt = a.ntape, ns = nt - 1;
   const bool overlap = nt > 1;
-  if (overlap) HIP_OK(hipEventRecord(a.ev_fwd, stream));  // the checkpoints are written
   auto rs_of = [&](int c) { return overlap ? a.rec_stream[c % ns] : stream; };
   auto segment_set = [&](Tape& tp, int kc, int q) -> BucketSet& {
     return tp.sbs.size() >= (size_t)kc + 2 ? tp.sbs[q] : bs[q % 3];
   };
-  // recompute segment c from its checkpoint into tape c % nt, recording the tape
-  auto recompute = [&](int c) {
+  // recompute segment c from its checkpoint into tape c % nt, recording the tape; ready: an
+  // event on the engine's stream after which checkpoint c is written
+  auto recompute = [&](int c, hipEvent_t ready) {
     const int t0 = c * K, kc = std::min(K, T - t0);
     Tape& tp = a.tp[c % nt];
     hipStream_t rs = rs_of(c);
     if (overlap) {
-      HIP_OK(hipStreamWaitEvent(rs, a.ev_fwd, 0));
+      HIP_OK(hipStreamWaitEvent(rs, ready, 0));
       if (c + nt <= C - 1) HIP_OK(hipStreamWaitEvent(rs, a.ev_rev[c % nt], 0));  // segment c + nt is done with it
     }
     HIP_OK(hipMemcpyAsync(tp.xycs, a.ck_xycs + c * N, N * sizeof(float4), hipMemcpyDeviceToDevice, rs));
@@ -248,17 +233,41 @@ int32_t Engine::adjoint(int T, const float* nz, size_t stride, int K, const tiss
     }
     if (overlap) HIP_OK(hipEventRecord(a.ev_rev[c % nt], stream));
   };
+  // forward, with a checkpoint at the start of every segment
+  double fwd_ms = 0.0;
+  int32_t status = TISSUE_OK;
+  for (int c = 0; c < C; ++c) {
+    const int t0 = c * K, kc = std::min(K, T - t0);
+    HIP_OK(hipMemcpyAsync(a.ck_xycs + c * N, xycs[sphase], N * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+    HIP_OK(hipMemcpyAsync(a.ck_th + c * N, th[sphase], N * sizeof(float), hipMemcpyDeviceToDevice, stream));
+    HIP_OK(hipMemcpyAsync(a.ck_vid + c * N, vid[sphase], N * sizeof(int), hipMemcpyDeviceToDevice, stream));
+    if (overlap && c >= C - ns) {
+      HIP_OK(hipEventRecord(a.ev_ck[c % nt], stream));
+      recompute(c, a.ev_ck[c % nt]);
+    }
+    const int32_t st = run_shape(s, kc, nz, stride, t0);
+    fwd_ms += last_ms;
+    if (st != TISSUE_OK && status == TISSUE_OK) status = st;
+  }
+
+
+  // reverse
+  HIP_OK(hipEventRecord(ev0, stream));
+  HIP_OK(launch_adj_seed(R, d_reps, xycs[sphase], L, d_halt, a.lr[0], a.lt[0], a.out_seed, stream));
+  HIP_OK(hipMemsetAsync(a.acc, 0, kAdjAcc * N * sizeof(double), stream));
+  HIP_OK(hipMemsetAsync(a.sexp, 0, 2 * R * sizeof(int), stream));
+  HIP_OK(hipMemsetAsync(a.mx, 0, 2 * R * sizeof(unsigned), stream));
   if (overlap) {
-    for (int k = 0; k < ns && C - 1 - k >= 0; ++k) recompute(C - 1 - k);
+    HIP_OK(hipEventRecord(a.ev_fwd, stream));  // every checkpoint is written
     for (int c = C - 1; c >= 0; --c) {
       reverse(c);
-      if (c - ns >= 0) recompute(c - ns);
+      if (c - ns >= 0) recompute(c - ns, a.ev_fwd);
     }
   } else {
-    recompute(C - 1);
+    recompute(C - 1, nullptr);
     for (int c = C - 1; c >= 0; --c) {
       reverse(c);
-      if (c >= 1) recompute(c - 1);
+      if (c >= 1) recompute(c - 1, nullptr);
     }
   }
   HIP_OK(launch_adj_reduce(R, d_reps, ntot, a.acc, a.out_red, stream));
