@@ -91,7 +91,8 @@ struct StepOut {
 
 // table entries per lane loaded together in the neighbour walk (scan phase, and the FP64 redo)
 constexpr int kRowChunk = 4;
-constexpr int kScanChunk = 8;
+constexpr int kScanChunk = 8;  // measured on MI355X: 4 and 16 are slower
+static_assert(32 % kScanChunk == 0, "scan chunks must tile the 32-entry window");
 
 // Global-memory neighbour source: the previous step's SoA state and vertex buckets.
 struct GlobalSource {
