@@ -108,6 +108,57 @@ __device__ __forceinline__ int rescale_exponent(unsigned mx_bits) {
   return s > 0 ? s : 0;
 }
 
+// particle j's factors of step t's reverse, from its tape entry and its incoming adjoint (scaled
+// by 2^-E, then by down = 2^-s_up): the factors its neighbours gather, its own terms of the new
+// adjoint, and what the accumulators take.
+struct AdjLocal {
+  float2 lF, lsc;          // d loss / d F, (d loss / d sum sin, d loss / d sum cos)
+  float lrx, lry, lth;     // the particle's own terms of the new adjoint
+  float ls, m, ltn, wv, dvx, dvy;
+};
+
+__device__ __forceinline__ AdjLocal adj_local(const StepParams& prm, const LossDev& L, int n_r, float down, float src,
+                                              float4 me, float4 f, int quarter, float2 lr_s, float lt_s) {
+  AdjLocal o;
+  const float2 lp = rot_t(quarter, make_float2(lr_s.x * down, lr_s.y * down));
+  o.ltn = lt_s * down;
+  o.m = sqrtf(f.x * f.x + f.y * f.y);
+  const float s = prm.v0 + prm.mu * o.m;
+  const float rdx = me.z * s, rdy = me.w * s;
+  float ux, uy, jxx, jxy, jyx, jyy;
+  field_eval(L, me.x, me.y, ux, uy, jxx, jxy, jyx, jyy);
+  o.wv = L.w_velocity / ((float)L.n_steps * (float)n_r);
+  o.dvx = rdx - ux;
+  o.dvy = rdy - uy;
+  const float sw = 2.f * o.wv * src;
+  const float lrdx = prm.dt * lp.x + sw * o.dvx, lrdy = prm.dt * lp.y + sw * o.dvy;
+  o.lrx = lp.x - sw * (jxx * o.dvx + jyx * o.dvy);
+  o.lry = lp.y - sw * (jxy * o.dvx + jyy * o.dvy);
+  o.ls = lrdx * me.z + lrdy * me.w;
+  const float lm = prm.mu * o.ls;
+  o.lF = o.m > 0.f ? make_float2(lm * f.x / o.m, lm * f.y / o.m) : make_float2(0.f, 0.f);
+  const float Q = f.z * f.z + f.w * f.w;
+  const float lmean = prm.rate * o.ltn;
+  o.lsc = Q > 0.f ? make_float2(lmean * f.z / Q, -lmean * f.w / Q) : make_float2(0.f, 0.f);
+  o.lth = (1.f - prm.rate) * o.ltn + s * (-lrdx * me.w + lrdy * me.z);
+  return o;
+}
+
+// parameter gradients and the velocity loss of particle g, accumulated in FP64 (true units)
+__device__ __forceinline__ void adj_accumulate(const StepParams& prm, const AdjLocal& o, float4 f, float theta,
+                                               double unscale, double* acc, size_t ntot, int g) {
+  double delta = atan2((double)f.w, (double)f.z) - (double)theta;
+  if (delta > kPi) delta -= 2.0 * kPi;
+  else if (delta <= -kPi) delta += 2.0 * kPi;
+  const double tau = prm.dt64 / prm.rate64;
+  acc[g] += unscale * (double)o.ls;                                            // d v0
+  acc[ntot + g] += unscale * (double)o.ls * (double)o.m;                       // d mu
+  if (prm.k64 != 0.0)                                                          // d k
+    acc[2 * ntot + g] += unscale * ((double)o.lF.x * f.x + (double)o.lF.y * f.y) / prm.k64;
+  acc[3 * ntot + g] += unscale * (double)o.ltn * delta * (-prm.dt64 / (tau * tau));  // d tau
+  acc[4 * ntot + g] += (double)o.wv * ((double)o.dvx * o.dvx + (double)o.dvy * o.dvy);  // loss
+}
+
 __global__ void k_adj_local(const ReplicaDesc* reps, const int* rid, int ntot, LossDev L, const unsigned* halt,
                             const int* sexp_in, int* sexp_out, const unsigned* mx_in, unsigned* mx_next,
                             const float4* xycs, const float* th, const float4* fs, const int* quarter,
@@ -131,47 +182,31 @@ __global__ void k_adj_local(const ReplicaDesc* reps, const int* rid, int ntot, L
     return;
   }
   const StepParams& prm = reps[r].prm;
-  const float down = ldexpf(1.f, -s_up);
-  const float src = ldexpf(1.f, -E);        // loss sources in the scaled units
-  const double unscale = ldexp(1.0, E);     // back to true units for the accumulators
-  const float4 me = xycs[g];
-  const float theta = th[g];
   const float4 f = fs[g];  // (Fx, Fy, sum cos, sum sin)
-  const float2 lr_s = lr_in[g];
-  const float2 lp = rot_t(quarter[g], make_float2(lr_s.x * down, lr_s.y * down));
-  const float ltn = lt_in[g] * down;
-  const float m = sqrtf(f.x * f.x + f.y * f.y);
-  const float s = prm.v0 + prm.mu * m;
-  const float rdx = me.z * s, rdy = me.w * s;
-  float ux, uy, jxx, jxy, jyx, jyy;
-  field_eval(L, me.x, me.y, ux, uy, jxx, jxy, jyx, jyy);
-  const float wv = L.w_velocity / ((float)L.n_steps * (float)reps[r].n);
-  const float dvx = rdx - ux, dvy = rdy - uy;
-  const float sw = 2.f * wv * src;
-  const float lrdx = prm.dt * lp.x + sw * dvx, lrdy = prm.dt * lp.y + sw * dvy;
-  const float lrx = lp.x - sw * (jxx * dvx + jyx * dvy);
-  const float lry = lp.y - sw * (jxy * dvx + jyy * dvy);
-  const float ls = lrdx * me.z + lrdy * me.w;
-  const float lm = prm.mu * ls;
-  const float2 lF = m > 0.f ? make_float2(lm * f.x / m, lm * f.y / m) : make_float2(0.f, 0.f);
-  const float Q = f.z * f.z + f.w * f.w;
-  const float lmean = prm.rate * ltn;
-  const float2 lsc = Q > 0.f ? make_float2(lmean * f.z / Q, -lmean * f.w / Q) : make_float2(0.f, 0.f);
-  const float lth = (1.f - prm.rate) * ltn + s * (-lrdx * me.w + lrdy * me.z);
-  gF[g] = lF;
-  gSC[g] = lsc;  // (d loss / d sum sin, d loss / d sum cos)
-  part[g] = make_float4(lrx, lry, lth, 0.f);
-  // parameter gradients and the velocity loss, accumulated per particle in FP64
-  double delta = atan2((double)f.w, (double)f.z) - (double)theta;
-  if (delta > kPi) delta -= 2.0 * kPi;
-  else if (delta <= -kPi) delta += 2.0 * kPi;
-  const double tau = prm.dt64 / prm.rate64;
-  acc[g] += unscale * (double)ls;                                            // d v0
-  acc[(size_t)ntot + g] += unscale * (double)ls * (double)m;                 // d mu
-  if (prm.k64 != 0.0)                                                        // d k
-    acc[2 * (size_t)ntot + g] += unscale * ((double)lF.x * f.x + (double)lF.y * f.y) / prm.k64;
-  acc[3 * (size_t)ntot + g] += unscale * (double)ltn * delta * (-prm.dt64 / (tau * tau));  // d tau
-  acc[4 * (size_t)ntot + g] += (double)wv * ((double)dvx * dvx + (double)dvy * dvy);       // loss
+  const AdjLocal o = adj_local(prm, L, reps[r].n, ldexpf(1.f, -s_up), ldexpf(1.f, -E), xycs[g], f, quarter[g],
+                               lr_in[g], lt_in[g]);
+  gF[g] = o.lF;
+  gSC[g] = o.lsc;
+  part[g] = make_float4(o.lrx, o.lry, o.lth, 0.f);
+  adj_accumulate(prm, o, f, th[g], ldexp(1.0, E), acc, (size_t)ntot, g);
+}
+
+// each replica's max |lambda| over the wave: the wave's particles of one replica occupy a
+// contiguous range of lanes, so an inclusive max-scan restricted to that range leaves the max on
+// its last lane, which posts it (float bits of non-negative values order like the values;
+// replicas are concatenated, so r never decreases along the wave). Every lane of the wave calls it.
+__device__ __forceinline__ void post_replica_max(int r, float mx, unsigned* mx_out) {
+  const int me_lane = (int)__lane_id();
+  const int r_prev = __shfl_up(r, 1, 64), r_next = __shfl_down(r, 1, 64);
+  const unsigned long long heads = __ballot(me_lane == 0 || r_prev != r);
+  const unsigned long long upto = me_lane == 63 ? ~0ull : (2ull << me_lane) - 1ull;
+  const int first = 63 - __clzll(heads & upto);
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const float up = __shfl_up(mx, o, 64);
+    if (me_lane - o >= first) mx = fmaxf(mx, up);
+  }
+  if ((me_lane == 63 || r_next != r) && r >= 0 && mx > 0.f) atomicMax(&mx_out[r], __float_as_uint(mx));
 }
 
 template <int LANES>
@@ -225,21 +260,7 @@ __global__ void __launch_bounds__(256) k_adj_gather(MeshDev mesh, const ReplicaD
       mx = fmaxf(fabsf(o.x), fmaxf(fabsf(o.y), fabsf(ot)));  // fmaxf skips NaN
     }
   }
-  // each replica's max |lambda| over the wave: the wave's particles of one replica occupy a
-  // contiguous range of lanes, so an inclusive max-scan restricted to that range leaves the max
-  // on its last lane, which posts it (float bits of non-negative values order like the values)
-  // (replicas are concatenated, so r never decreases along the wave)
-  const int me_lane = (int)__lane_id();
-  const int r_prev = __shfl_up(r, 1, 64), r_next = __shfl_down(r, 1, 64);
-  const unsigned long long heads = __ballot(me_lane == 0 || r_prev != r);
-  const unsigned long long upto = me_lane == 63 ? ~0ull : (2ull << me_lane) - 1ull;
-  const int first = 63 - __clzll(heads & upto);
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const float up = __shfl_up(mx, o, 64);
-    if (me_lane - o >= first) mx = fmaxf(mx, up);
-  }
-  if ((me_lane == 63 || r_next != r) && r >= 0 && mx > 0.f) atomicMax(&mx_out[r], __float_as_uint(mx));
+  post_replica_max(r, mx, mx_out);
 }
 
 // one block per replica: sums of the per-particle accumulators
