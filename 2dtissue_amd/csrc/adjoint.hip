@@ -225,7 +225,7 @@ __global__ void __launch_bounds__(256) k_adj_gather(MeshDev mesh, const ReplicaD
     const ReplicaDesc& rep = reps[r];
     const StepParams& prm = rep.prm;
     const int nv = mesh.nv;
-    const BucketsIn b{bs.count + (size_t)r * nv, bs.slot + (size_t)r * nv * kSlots, bs.big + rep.off};
+    const BucketsIn b = replica_in(bs, r, nv, rep.off);
     const float2 mine = gF[g];
     float ax = 0.f, ay = 0.f, aS = 0.f, aC = 0.f;
     const int v = vid[g];

@@ -33,10 +33,19 @@ struct StepParams {
 // an FP32 speed carries ~1e-7 relative error, which a displacement D turns into ~1e-7 D absolute
 constexpr float kBigStep = 2.0f;
 
-// Vertex buckets of one step, read after k_bucket_finalize has run on them: count[v] particles;
-// up to kSlots of them in slot[v*kSlots ..] in ascending id order; a bucket with more holds its
-// ids, ascending, in big[slot[v*kSlots] ..] instead.
-struct BucketsIn { const int* count; const int* slot; const int* big; };
+// Vertex buckets of one step: count[v] particles. Sorted (after k_bucket_finalize): up to kSlots
+// of them in slot[v*kSlots ..] in ascending id order; a bucket with more holds its ids, ascending,
+// in big[slot[v*kSlots] ..] instead. Raw (no finalize pass): the first kSlots arrivals in
+// slot[v*kSlots ..] in arrival order, later ones in the (vertex, id) overflow list ovf[0 .. *ovf_n);
+// readers order them themselves. Either way a bucket is visited in ascending id order.
+struct BucketsIn {
+  const int* count;
+  const int* slot;
+  const int* big;
+  const int2* ovf;
+  const int* ovf_n;
+  int sorted;
+};
 // The same buckets while the step fills them: the first kSlots arrivals of a vertex take its
 // slots, later ones go to the (vertex, id) overflow list; the second arrival at a vertex also
 // lists the vertex in multi (the buckets the finalize pass must sort).
@@ -50,12 +59,41 @@ __device__ __forceinline__ void for_bucket(const B& b, int w, F&& f) {
   const int c = b.count[w];
   if (c == 0) return;
   const int* s = b.slot + (size_t)w * kSlots;
-  if (c <= kSlots) {
-    for (int q = 0; q < c; ++q) f(s[q]);
+  if (b.sorted) {
+    if (c <= kSlots) {
+      for (int q = 0; q < c; ++q) f(s[q]);
+      return;
+    }
+    const int* g = b.big + s[0];
+    for (int q = 0; q < c; ++q) f(g[q]);
     return;
   }
-  const int* g = b.big + s[0];
-  for (int q = 0; q < c; ++q) f(g[q]);
+  if (c <= kSlots) {  // raw: the slots in registers, sorted by the network (8 slots are in bounds)
+    const int4 s0 = reinterpret_cast<const int4*>(s)[0], s1 = reinterpret_cast<const int4*>(s)[1];
+    int a[kSlots] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
+#pragma unroll
+    for (int q = 0; q < kSlots; ++q) a[q] = q < c ? a[q] : 0x7fffffff;
+    sort8(a);
+    for (int q = 0; q < c; ++q) f(a[q]);
+    return;
+  }
+  // raw and crowded (rare for the systems that skip the finalize pass): the next id each time is
+  // the least one above the previous among the slots and the bucket's overflow entries
+  const int n_ovf = *b.ovf_n;
+  int prev = -1;
+  for (int k = 0; k < c; ++k) {
+    int best = 0x7fffffff;
+    for (int q = 0; q < kSlots; ++q) {
+      const int id = s[q];
+      best = (id > prev && id < best) ? id : best;
+    }
+    for (int o = 0; o < n_ovf; ++o) {
+      const int2 e = b.ovf[o];
+      best = (e.x == w && e.y > prev && e.y < best) ? e.y : best;
+    }
+    f(best);
+    prev = best;
+  }
 }
 
 // force coefficient Fij / d for a pair at table distance d < 2 sigma
@@ -114,7 +152,12 @@ struct GlobalSource {
   __device__ __forceinline__ void each(int w, const Head& h, F&& f) const {
     if (h.c == 0) return;
     if (h.c == 1) { f(h.s0); return; }
-    if (h.c == 2) { f(h.s0); f(h.s1); return; }  // sorted by the finalize pass
+    if (h.c == 2) {  // in order when sorted by the finalize pass; else ordered here
+      const int lo = b.sorted ? h.s0 : min(h.s0, h.s1), hi = b.sorted ? h.s1 : max(h.s0, h.s1);
+      f(lo);
+      f(hi);
+      return;
+    }
     for_bucket(b, w, f);
   }
   __device__ __forceinline__ float4 pos(int j) const { return xycs[j]; }

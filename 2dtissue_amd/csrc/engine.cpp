@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstdlib>
 #include <mutex>
 #include <numeric>
 #include <thread>
@@ -63,6 +64,17 @@ Engine::Engine(tissue_mesh* m, int R_, const int* n_, const tissue_params* p_) {
     }
     if (tot > INT_MAX / 16) throw ApiError(TISSUE_E_INVALID_ARG, "too many particles");
     ntot = (int)tot;
+    // replicas of a few thousand particles, at most one per two vertices, leave their vertex
+    // buckets raw: the finalize pass would cost a kernel boundary per step, and the readers order
+    // the few particles of each bucket themselves (the same visiting order, so the same results).
+    // Denser or larger systems pile particles onto vertices (the exit snapping), and a crowded raw
+    // bucket costs each reader a scan of the overflow list. TISSUE_SORTED_BUCKETS = 1 / 0 forces
+    // the choice.
+    raw_buckets = nmax <= kRawBucketsMax && 2L * nmax <= md.nv;
+    if (const char* e = std::getenv("TISSUE_SORTED_BUCKETS")) {
+      if (e[0] == '1') raw_buckets = false;
+      if (e[0] == '0') raw_buckets = true;
+    }
     HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIP_OK(hipEventCreate(&ev0));
     HIP_OK(hipEventCreate(&ev1));
@@ -222,6 +234,7 @@ void Engine::get_outputs(float* ovx, float* ovy, float* r3d, int32_t* oface, int
 
 void Engine::alloc_bucket_set(BucketSet& b) {
   const size_t cells = (size_t)R * md.nv;
+  b.sorted = raw_buckets ? 0 : 1;
   b.count = dalloc<int>(cells);
   b.slot = dalloc<int>(cells * tdev::kSlots);
   b.ovf_n = dalloc<int>(R);
@@ -258,7 +271,8 @@ void Engine::bucket_state_into(const int* v, BucketSet& b, hipStream_t st) {
   if (!d_tiles) ensure_tiles(resolve_tiled());
   clear_bucket_set(b, st);
   HIP_OK(launch_insert(ntot, v, d_rid, d_reps, md.nv, b, st));
-  HIP_OK(launch_bucket_finalize(std::get<1>(tiles_key), d_reps, d_tiles, n_phys, md.nv, R == 1 ? ntot : 0, b, st));
+  if (!raw_buckets)
+    HIP_OK(launch_bucket_finalize(std::get<1>(tiles_key), d_reps, d_tiles, n_phys, md.nv, R == 1 ? ntot : 0, b, st));
 }
 
 void Engine::ensure_buckets() {
@@ -487,7 +501,9 @@ void Engine::enqueue(const Shape& s, int n_steps, const float* nz, size_t stride
         if (rec)
           HIP_OK(launch_record(ntot, d_row0, t, rec->every, rec->depth, io.xycs_out, r3x, r3y, r3z, nbr, rec->d_ring,
                                rec->d_ring_nbr, stream));
-        HIP_OK(launch_bucket_finalize(s.block, d_reps, d_tiles, n_phys, md.nv, R == 1 ? ntot : 0, bs[(bp + 1) % 3], stream));
+        if (!raw_buckets)
+          HIP_OK(launch_bucket_finalize(s.block, d_reps, d_tiles, n_phys, md.nv, R == 1 ? ntot : 0, bs[(bp + 1) % 3],
+                                        stream));
         sp = 1 - sp;
         bp = (bp + 1) % 3;
       }

@@ -208,8 +208,9 @@ int32_t Engine::adjoint(int T, const float* nz, size_t stride, int K, const tiss
       io.tape_q = tp.q + q * N;
       HIP_OK(launch_step_tiled(s.lanes, s.block, true, md, d_reps, d_tiles, n_phys, segment_set(tp, kc, q),
                                segment_set(tp, kc, q + 1), segment_set(tp, kc, q + 2), io, rs));
-      HIP_OK(launch_bucket_finalize(s.block, d_reps, d_tiles, n_phys, md.nv, R == 1 ? ntot : 0,
-                                    segment_set(tp, kc, q + 1), rs));
+      if (!raw_buckets)
+        HIP_OK(launch_bucket_finalize(s.block, d_reps, d_tiles, n_phys, md.nv, R == 1 ? ntot : 0,
+                                      segment_set(tp, kc, q + 1), rs));
     }
     if (overlap) HIP_OK(hipEventRecord(a.ev_rec[c % nt], rs));
   };

@@ -32,10 +32,6 @@ __device__ __forceinline__ void bucket_insert(const BucketsOut& b, int v, int i)
   if (old == 1) b.multi[atomicAdd(b.multi_n, 1)] = v;  // the bucket needs sorting
 }
 
-// buckets of replica r within a set
-__device__ __forceinline__ BucketsIn replica_in(const BucketSet& s, int r, int nv, int off) {
-  return {s.count + (size_t)r * nv, s.slot + (size_t)r * nv * kSlots, s.big + off};
-}
 __device__ __forceinline__ BucketsOut replica_out(const BucketSet& s, int r, int nv, int off) {
   return {s.count + (size_t)r * nv, s.slot + (size_t)r * nv * kSlots, s.ovf_n + r, s.ovf + off, s.multi_n + r,
           s.multi + off};

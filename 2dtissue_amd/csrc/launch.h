@@ -39,7 +39,14 @@ struct BucketSet {
   int* multi;
   int* big_n;
   int* big;
+  int sorted;  // 1: readers see it after k_bucket_finalize; 0: raw (readers order each bucket)
 };
+
+// buckets of replica r (particles [off, ...)) within a set, as the step's readers see them
+__device__ __forceinline__ tdev::BucketsIn replica_in(const BucketSet& s, int r, int nv, int off) {
+  return {s.count + (size_t)r * nv, s.slot + (size_t)r * nv * tdev::kSlots, s.big + off, s.ovf + off, s.ovf_n + r,
+          s.sorted};
+}
 
 // everything one step of the tiled kernel reads and writes besides the mesh and the buckets
 struct TiledIO {
