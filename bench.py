@@ -74,6 +74,14 @@ def measured(workload: str | None, launch: dict | None):
     return m, d.get("source_commit")
 
 
+def step_kernels(nmax: int, uv_vertices: int, what: str = "one step") -> str:
+    """The kernels of one tiled step: replicas of up to 8,192 particles at most one per two vertices
+    keep their vertex buckets raw (the library's rule, engine.cpp), so no finalize pass runs."""
+    if nmax <= 8192 and 2 * nmax <= uv_vertices:
+        return f"k_step_tiled ({what}; raw vertex buckets, no finalize pass)"
+    return f"k_step_tiled + k_bucket_finalize ({what})"
+
+
 def roofline(particles: int, avg_launch_s: float, kernel: str, steps_per_launch: int = 1,
              workload: str | None = None, launch: dict | None = None) -> dict:
     """HBM roofline of one step: algorithmic bytes (64 B per particle-step: the state read and
@@ -204,8 +212,8 @@ def batched(T, mesh, steps, n_replicas=256, plain_sample=16):
                elapsed_s=elapsed, graph_ms=run_ms, status=code, replicas_flagged=int((flags != 0).sum()),
                launch=shape,
                roofline=roofline(b.ntot, run_ms / 1e3 / steps,
-                                 "k_step_tiled + k_bucket_finalize (one step, all replicas)" if shape["mode"] == 1
-                                 else "k_resident (one step, all replicas)",
+                                 step_kernels(int(np.max(sizes)), mesh.info()["nv"], "one step, all replicas")
+                                 if shape["mode"] == 1 else "k_resident (one step, all replicas)",
                                  workload="batched" if shape["mode"] == 1 else None, launch=shape))
     b.close()
     # the same replicas, each launched alone through the plain per-step kernel
@@ -289,7 +297,7 @@ def large(T, steps=500, adj_steps=200, cpu_threads=0):
     el = time.perf_counter() - t0
     shape = sim.launch()
     fwd = dict(value=n * steps / el, unit="particle-steps/s", status=code, elapsed_s=el, launch=shape,
-               roofline=roofline(n, sim.last_step_ms() / 1e3 / steps, "k_step_tiled + k_bucket_finalize (one step)",
+               roofline=roofline(n, sim.last_step_ms() / 1e3 / steps, step_kernels(n, info["nv"]),
                                  workload="large", launch=shape))
     sim.close()
     cpu = None
@@ -409,7 +417,7 @@ def main():
             "particles": n, "faces": tables["nf"], "uv_vertices": tables["nv"], "status": main_res["code"],
             "flags": main_res["flags"],
         },
-        "roofline": roofline(n, main_res["kernel_ms"] / 1e3 / args.steps, "k_step_tiled + k_bucket_finalize (one step)",
+        "roofline": roofline(n, main_res["kernel_ms"] / 1e3 / args.steps, step_kernels(n, tables["nv"]),
                              workload="plain", launch=main_res["launch"]),
         "valu": valu_axis(ps, main_res["pairs"], info["n_cell_faces"] / info["grid"] ** 2,
                           (measured("plain", main_res["launch"]) or ({},))[0].get("valu_issue_frac")),
