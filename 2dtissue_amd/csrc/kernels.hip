@@ -202,13 +202,22 @@ __global__ void __launch_bounds__(kStepMaxBlock) k_bucket_finalize(const Replica
       int* buf = lds[wave];
       if (lane < kSlots) buf[lane] = sl[lane];
       int fill = kSlots;
-      for (int q0 = 0; q0 < novf; q0 += 64) {
-        const int q = q0 + lane;
-        const int2 e = q < novf ? ovf[q] : make_int2(-1, 0);
-        const bool m = e.x == w;
-        const unsigned long long bm = __ballot(m);
-        if (m) buf[fill + __popcll(bm & ((1ull << lane) - 1ull))] = e.y;
-        fill += __popcll(bm);
+      // four chunks of the overflow list loaded together (one load latency per 256 entries), then
+      // compacted in order
+      for (int q0 = 0; q0 < novf; q0 += 256) {
+        int2 e[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int q = q0 + 64 * u + lane;
+          e[u] = q < novf ? ovf[q] : make_int2(-1, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const bool m = e[u].x == w;
+          const unsigned long long bm = __ballot(m);
+          if (m) buf[fill + __popcll(bm & ((1ull << lane) - 1ull))] = e[u].y;
+          fill += __popcll(bm);
+        }
       }
       if (lane < 4) buf[cw + lane] = 0x7fffffff;  // pad to a multiple of four (never below an id)
       wave_sync();
