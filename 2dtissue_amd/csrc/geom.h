@@ -194,7 +194,7 @@ __host__ __device__ inline bool seam_wrap(double ax, double ay, double& px, doub
 
 // The border tests and the crossing rewritten for the lane-parallel loop: the same roundings as
 // border_test and cross_border, with the choices between borders made by exact negations and
-// sign-bit flips instead of selects, so one crossing is about 100 instructions instead of 140.
+// sign-bit flips instead of selects, so one crossing is 114 instructions instead of 136 (gfx950).
 // The identities used (round to nearest is symmetric): fl(-0.0 - x) = -x bit for bit (zero
 // signs included), fl(1 - x) = omx, fl(2 - x) as the reference forms it.
 //
