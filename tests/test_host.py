@@ -198,6 +198,17 @@ def test_mesh_errors(T):
         T.Mesh.ellipsoid(-1.0, 1.0, 1.0, 4)
     with pytest.raises(T.TissueError):
         T.Mesh.load_off("/nonexistent/mesh.off")
+    m = T.Mesh.ellipsoid(2.0, 1.0, 1.0, 6)
+    with pytest.raises(T.TissueError):
+        m.build_tables(0.1, where=7)  # no such place
+    with pytest.raises(T.TissueError):
+        m.build_tables(-0.1, where=T.BUILD_HOST)
+    m.build_tables(0.1, where=T.BUILD_HOST)
+    assert m.info()["nnz"] > 0
+    try:  # the GPU march is an error exactly when no GPU is visible
+        T.Mesh.ellipsoid(2.0, 1.0, 1.0, 6).build_tables(0.1, where=T.BUILD_GPU)
+    except T.TissueError as e:
+        assert "no GPU" in str(e)
 
 
 def restated_init(T, tb, seed, n):
