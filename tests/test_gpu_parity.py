@@ -9,7 +9,7 @@ import parity as P
 pytestmark = pytest.mark.gpu
 
 
-def run_teacher_forced(T, O, cfg, steps, lanes=0, mode=0):
+def run_teacher_forced(T, O, cfg, steps, lanes=0, mode=0, sparse=False):
     m, tb = P.mesh_and_tables(cfg["freq"], cfg["sigma"])
     om = O.Mesh(tb)
     p = dict(P.PHYS, sigma=cfg["sigma"])
@@ -25,7 +25,7 @@ def run_teacher_forced(T, O, cfg, steps, lanes=0, mode=0):
         assert sim.step(1, noise[t]) == T.OK
         s_out = sim.get_state()
         out = sim.get_outputs()
-        fails, _ = P.compare_step(O, om, p, s_in, s_out, out, nb, noise[t].astype(np.float64))
+        fails, _ = P.compare_step(O, om, p, s_in, s_out, out, nb, noise[t].astype(np.float64), sparse)
         for k, v in fails.items():
             total[k] = total.get(k, 0) + v
         if any(fails.values()):
@@ -43,6 +43,14 @@ def test_c1_teacher_forced_50_steps(T, O, lanes, mode):
 @pytest.mark.parametrize("mode", [1, 2])
 def test_c2_teacher_forced_20_steps(T, O, mode):
     total = run_teacher_forced(T, O, P.C2, 20, 0, mode)
+    assert all(v == 0 for v in total.values()), total
+
+
+def test_c2_teacher_forced_1000_steps(T, O):
+    """The whole 1,000-step run of the 2,000-particle system, every step checked against the
+    oracle's O(N) step (bitwise the dense one): exact neighbour sets, counts, vertex and face ids,
+    positions, velocities and headings within 1e-5."""
+    total = run_teacher_forced(T, O, P.C2, 1000, 0, 1, sparse=True)
     assert all(v == 0 for v in total.values()), total
 
 
