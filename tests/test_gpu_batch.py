@@ -220,3 +220,50 @@ def test_set_state_rejects_bad_input(T):
         s.set_state([0.5, 0.5, 0.2], [0.5, float("nan"), 0.5], [0, 0, 0])
     with pytest.raises(T.TissueError):
         s.step(1)  # no state yet
+
+
+def test_ensemble_workload_equals_solo_runs(T):
+    """The ensemble workload at full size: 256 replicas (sizes U[100, 4000], tau log-uniform in
+    [0.1, 10], noise std U[0, 0.5], v0 U[0.5, 2]; parameter seed 3, init seed 2000 + r, noise seed
+    1000 + r), 500 steps in one run at the autotuned mode and launch shape. Eight replicas spread
+    over the size range, each run alone with the same lanes per particle, give its final state and
+    both order-parameter histories. (Another lane count splits each particle's neighbour sums into
+    other partial sums: its trajectories agree with these within the parity tolerance per step, not
+    bit for bit.)"""
+    sigma = 0.05
+    m, _ = P.mesh_and_tables(32, sigma)
+    R, steps = 256, 500
+    rng = np.random.default_rng(3)
+    sizes = rng.integers(100, 4001, R)
+    tau = np.exp(rng.uniform(np.log(0.1), np.log(10.0), R))
+    std = rng.uniform(0.0, 0.5, R)
+    v0 = rng.uniform(0.5, 2.0, R)
+    ps = [T.params(sigma=sigma, v0=float(v0[r]), k=10.0, mu=1.0, tau=float(tau[r]), dt=0.02) for r in range(R)]
+    init_seeds = np.arange(R, dtype=np.uint64) + 2000
+    noise_seeds = np.arange(R, dtype=np.uint64) + 1000
+    b = T.Batch(m, sizes, ps)
+    b.init_random(init_seeds)
+    b.gen_noise(noise_seeds, std, steps)
+    b.autotune(20)
+    b.init_random(init_seeds)
+    assert b.run(steps, 0) == T.OK
+    flags, _ = b.flags()
+    assert not flags.any()
+    st = b.get_state()
+    shape = b.launch()
+    print("ensemble launch", shape, flush=True)
+    for r in np.argsort(sizes)[np.linspace(0, R - 1, 8).astype(int)]:
+        n, o = int(sizes[r]), int(b.off[r])
+        s = T.Sim(m, ps[r], n)
+        s.set_mode(shape["mode"])
+        s.set_launch(shape["lanes"], shape["block"])  # same lanes per particle: same summation order
+        s.init_random(int(init_seeds[r]))
+        s.stage_noise(T.gen_noise(int(noise_seeds[r]), float(std[r]), steps, n))
+        assert s.step_staged(steps, 0) == T.OK
+        ss = s.get_state()
+        for k in ("x", "y", "theta", "vid"):
+            assert np.array_equal(getattr(ss, k), getattr(st, k)[o:o + n]), (r, k)
+        (ra, pa), (rb, pb) = s.order(), b.order(int(r))
+        assert len(ra) == steps and np.array_equal(ra, rb) and np.allclose(pa, pb, rtol=0, atol=1e-12), r
+        s.close()
+    b.close()
