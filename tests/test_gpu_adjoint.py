@@ -145,10 +145,12 @@ def test_batch_adjoint_equals_solo(T):
     b.gen_noise(seeds, np.array([0.1, 0.2, 0.3]), steps)
     code, gs, dth, dx, dy = b.adjoint(steps, Lc, 0, 4)
     assert code == T.OK
+    shape = b.launch()
     for r, n in enumerate(sizes):
         o = int(b.off[r])
         s = T.Sim(m, ps[r], n)
         s.set_mode(1)
+        s.set_launch(shape["lanes"], shape["block"])  # same lanes per particle: same sums
         s.init_random(int(seeds[r]))
         s.stage_noise(T.gen_noise(int(seeds[r]), [0.1, 0.2, 0.3][r], steps, n))
         c2, g, sth, sx, sy = s.adjoint(steps, Lc, 0, 4)

@@ -49,12 +49,15 @@ def test_large_surface_batched_adjoint(T, O):
     L = T.loss(0.8, 1.0, 0.1, T.target_field(4))
     code, gs, dth, _, _ = b.adjoint(steps, L, 0, 5)
     assert code == T.OK
+    shape = b.launch()
+    fin = b.get_state()
     modes = T.target_field(4)
     comp, amp, kx, ky, ph = (np.array(v) for v in zip(*modes))
     Lo = O.Loss(0.8, 1.0, 0.1, component=comp, amp=amp, kx=kx, ky=ky, phase=ph)
     for r in (0, 17, 63):
         s = T.Sim(m, ps[r], n)
         s.set_mode(T.MODE_STEP_KERNEL)
+        s.set_launch(shape["lanes"], shape["block"])  # the batch's lanes per particle: its trajectory
         s.init_random(int(seeds[r]))
         eta = T.gen_noise(int(seeds[r]), 0.1, steps, n)
         xs, ys, ths, vids = [], [], [], []
@@ -65,11 +68,12 @@ def test_large_surface_batched_adjoint(T, O):
                 assert s.step(1, eta[t]) == T.OK
         states = dict(x=np.array(xs, np.float64), y=np.array(ys, np.float64), theta=np.array(ths, np.float64),
                       vid=np.array(vids, np.int32))
+        o = r * n
+        assert np.array_equal(xs[-1], fin.x[o:o + n]) and np.array_equal(vids[-1], fin.vid[o:o + n]), r
         pr = dict(sigma=cfg["sigma"], v0=1.0, k=10.0, mu=1.0, tau=float(taus[r]), dt=0.02)
         ref = O.adjoint_reverse(om, pr, states, Lo)
         for key in ("d_tau", "d_mu", "d_v0"):
             assert abs(gs[r][key] - ref[key]) <= 1e-4 * abs(ref[key]), (r, key, gs[r][key], ref[key])
-        o = r * n
         d = dth[o:o + n]
         assert np.linalg.norm(d - ref["d_theta0"]) <= 1e-4 * np.linalg.norm(ref["d_theta0"]), r
         s.close()
