@@ -11,7 +11,10 @@ import parity as P
 pytestmark = pytest.mark.gpu
 
 
-def test_large_surface_teacher_forced(T, O):
+@pytest.mark.parametrize("first", [2, 498])
+def test_large_surface_teacher_forced(T, O, first):
+    """Two steps teacher-forced, early (after leaving the initial placement) and at the end of the
+    500-step run (after hundreds of long flights have piled particles onto the chart's corners)."""
     cfg = P.C5
     m, tb = P.sphere_and_tables(cfg["freq"], cfg["amp"], cfg["sigma"])
     om = O.Mesh(tb)
@@ -19,10 +22,11 @@ def test_large_surface_teacher_forced(T, O):
     p = dict(P.PHYS, sigma=cfg["sigma"])
     sim = T.Sim(m, T.params(**p), cfg["n"])
     sim.init_random(cfg["seed"])
-    noise = T.gen_noise(cfg["noise_seed"], 0.1, 4, cfg["n"])
-    assert sim.step(2, noise[:2]) == T.OK  # leave the initial placement first
+    noise = T.gen_noise(cfg["noise_seed"], 0.1, first + 2, cfg["n"])
+    sim.stage_noise(noise)
+    assert sim.step_staged(first, 0) == T.OK
     total = {}
-    for t in range(2, 4):
+    for t in range(first, first + 2):
         s_in = sim.get_state()
         nb = sim.neighbor_lists()
         assert sim.step(1, noise[t]) == T.OK
