@@ -130,6 +130,10 @@ struct StepOut {
 // table entries per lane loaded together in the neighbour walk (scan phase, and the FP64 redo)
 constexpr int kRowChunk = 4;
 constexpr int kScanChunk = 8;  // measured on MI355X: 4 and 16 are slower
+// lanes per particle from which the walk loads bucket heads in its scan (measured on MI355X: 1.7 %
+// faster at 16 lanes, 1 % slower at 4 lanes with chunks of 4), and its chunk of entries per lane
+constexpr int kHeadScanLanes = 8;
+constexpr int kHeadChunk = 2;
 static_assert(32 % kScanChunk == 0, "scan chunks must tile the 32-entry window");
 
 // Global-memory neighbour source: the previous step's SoA state and vertex buckets.
@@ -223,58 +227,94 @@ __device__ __forceinline__ StepOut particle_step(const MeshDev& mesh, const Step
   const double kc64 = prm.k64 / prm.two_sigma64;  // Fij / d = kc (d - 2 sigma) / d, to a few ulps
   int na = 0, nb = 0;
   const int rs = mesh.row_ptr[v], re = mesh.row_ptr[v + 1];
-  // Each lane walks its entries rs + lane + m * LANES, m = 0, 1, .., in order, in windows of 32:
-  //  (1) scan: the window's table entries and their bucket counts are loaded kScanChunk at a time
-  //      (one load latency per chunk), counted-only entries (2 sigma <= d <= 2.4 sigma) add their
-  //      count to nb, and near entries whose bucket holds particles are marked;
-  //  (2) visit: only the marked entries, in order, set up their pair coefficient and add their
-  //      particles. Most table entries have empty buckets, so the lanes of a wave loop over a few
-  //      occupied entries instead of paying the pair set-up for every entry; the order, and so
-  //      every sum, is that of a walk over all entries.
-  bool done = false;
-  for (int wbase = rs + lane; !done && wbase < re; wbase += 32 * LANES) {
-    unsigned mark = 0u;
-    for (int m0 = 0; m0 < 32 && !done; m0 += kScanChunk) {
-      int2 ent[kScanChunk];
-      int cnt[kScanChunk];
+  // pair set-up of a near entry (vertex w at distance d, bucket head h) and its particles
+  auto visit = [&](int w, float d, const typename Src::Head& h) __attribute__((always_inline)) {
+    const double dd = d == 0.0f ? 0.001 : (double)d;
+    const double g64 = (kc64 * (dd - prm.two_sigma64)) / dd;
+    const bool counted = d != 0.0f;
+    src.each(w, h, [&](int j) {
+      const float4 q = src.pos(j);
+      if (j != i) {
+        Fx64 += g64 * ((double)me.x - (double)q.x);
+        Fy64 += g64 * ((double)me.y - (double)q.y);
+      }
+      Sx += q.z;
+      Sy += q.w;
+      ++na;
+      if (counted) ++nb;
+    });
+  };
+  // Each lane walks its entries rs + lane + m * LANES, m = 0, 1, .., in order.
+  if constexpr (LANES >= kHeadScanLanes) {
+    // At most a few entries per lane: the scan loads each near entry's bucket head (count and
+    // first slots) together with the entries' counts, and the lane visits its occupied entries
+    // from registers: table entry -> bucket head -> particles, no reload of marked entries.
+    for (int base = rs + lane; base < re; base += kHeadChunk * LANES) {
+      int2 ent[kHeadChunk];
+      typename Src::Head hd[kHeadChunk];
 #pragma unroll
-      for (int k = 0; k < kScanChunk; ++k) {
-        const int e = wbase + (m0 + k) * LANES;
+      for (int k = 0; k < kHeadChunk; ++k) {
+        const int e = base + k * LANES;
         ent[k] = e < re ? mesh.nbr[e] : make_int2(0, 0x7f800000);  // +inf ends the walk
       }
 #pragma unroll
-      for (int k = 0; k < kScanChunk; ++k) cnt[k] = __int_as_float(ent[k].y) <= prm.t_count ? src.count(ent[k].x) : 0;
+      for (int k = 0; k < kHeadChunk; ++k) {
+        const float d = __int_as_float(ent[k].y);
+        if (d <= prm.t_near) hd[k] = src.head(ent[k].x);
+        else if (d <= prm.t_count) hd[k].c = src.count(ent[k].x);
+      }
+      bool stop = false;
 #pragma unroll
-      for (int k = 0; k < kScanChunk; ++k) {
+      for (int k = 0; k < kHeadChunk; ++k) {
         const float d = __int_as_float(ent[k].y);
         if (d > prm.t_count) {  // rows ascend in distance
-          done = true;
+          stop = true;
           break;
         }
-        if (d > prm.t_near) nb += cnt[k];  // counted only
-        else if (cnt[k] > 0) mark |= 1u << (m0 + k);
+        if (d > prm.t_near) nb += hd[k].c;  // counted only
+        else if (hd[k].c > 0) visit(ent[k].x, d, hd[k]);
       }
+      if (stop) break;
     }
-    while (mark) {
-      const int m = __builtin_ctz(mark);
-      mark &= mark - 1u;
-      const int2 en = mesh.nbr[wbase + m * LANES];
-      const typename Src::Head head = src.head(en.x);
-      const float d = __int_as_float(en.y);
-      const double dd = d == 0.0f ? 0.001 : (double)d;
-      const double g64 = (kc64 * (dd - prm.two_sigma64)) / dd;
-      const bool counted = d != 0.0f;
-      src.each(en.x, head, [&](int j) {
-        const float4 q = src.pos(j);
-        if (j != i) {
-          Fx64 += g64 * ((double)me.x - (double)q.x);
-          Fy64 += g64 * ((double)me.y - (double)q.y);
+  } else {
+    // Windows of 32 entries per lane:
+    //  (1) scan: the window's table entries and their bucket counts are loaded kScanChunk at a
+    //      time (one load latency per chunk), counted-only entries (2 sigma <= d <= 2.4 sigma) add
+    //      their count to nb, and near entries whose bucket holds particles are marked;
+    //  (2) visit: only the marked entries, in order, set up their pair coefficient and add their
+    //      particles. Most table entries have empty buckets, so the lanes of a wave loop over a
+    //      few occupied entries instead of paying the pair set-up for every entry; the order, and
+    //      so every sum, is that of a walk over all entries.
+    bool done = false;
+    for (int wbase = rs + lane; !done && wbase < re; wbase += 32 * LANES) {
+      unsigned mark = 0u;
+      for (int m0 = 0; m0 < 32 && !done; m0 += kScanChunk) {
+        int2 ent[kScanChunk];
+        int cnt[kScanChunk];
+#pragma unroll
+        for (int k = 0; k < kScanChunk; ++k) {
+          const int e = wbase + (m0 + k) * LANES;
+          ent[k] = e < re ? mesh.nbr[e] : make_int2(0, 0x7f800000);  // +inf ends the walk
         }
-        Sx += q.z;
-        Sy += q.w;
-        ++na;
-        if (counted) ++nb;
-      });
+#pragma unroll
+        for (int k = 0; k < kScanChunk; ++k) cnt[k] = __int_as_float(ent[k].y) <= prm.t_count ? src.count(ent[k].x) : 0;
+#pragma unroll
+        for (int k = 0; k < kScanChunk; ++k) {
+          const float d = __int_as_float(ent[k].y);
+          if (d > prm.t_count) {  // rows ascend in distance
+            done = true;
+            break;
+          }
+          if (d > prm.t_near) nb += cnt[k];  // counted only
+          else if (cnt[k] > 0) mark |= 1u << (m0 + k);
+        }
+      }
+      while (mark) {
+        const int m = __builtin_ctz(mark);
+        mark &= mark - 1u;
+        const int2 en = mesh.nbr[wbase + m * LANES];
+        visit(en.x, __int_as_float(en.y), src.head(en.x));
+      }
     }
   }
   Fx64 = group_sum<LANES>(Fx64);
