@@ -140,6 +140,7 @@ struct tissue_mesh {
   int32_t* d_faces = nullptr;
   int32_t* d_cell_start = nullptr;
   float4* d_cell_tri = nullptr;
+  float4* d_face_pack = nullptr;
   double* d_cdf = nullptr;  // cumulative face areas (area-uniform initialisation)
   int32_t* d_row_ptr = nullptr;
   int2* d_nbr = nullptr;
@@ -149,7 +150,7 @@ struct tissue_mesh {
     int cur = 0;
     (void)hipGetDevice(&cur);
     (void)hipSetDevice(device);
-    dfree(d_uv); dfree(d_xyz); dfree(d_faces); dfree(d_cell_start); dfree(d_cell_tri); dfree(d_cdf); dfree(d_row_ptr); dfree(d_nbr);
+    dfree(d_uv); dfree(d_xyz); dfree(d_faces); dfree(d_cell_start); dfree(d_cell_tri); dfree(d_face_pack); dfree(d_cdf); dfree(d_row_ptr); dfree(d_nbr);
     (void)hipSetDevice(cur);
     dev_ready = false;
   }
@@ -180,6 +181,10 @@ struct tissue_mesh {
       for (size_t q = 0; q < m.cell_faces.size(); ++q)
         tdev::pack_cell_triangle(reinterpret_cast<const float*>(uv.data()), m.faces.data(), m.cell_faces[q], &tri[2 * q]);
       d_cell_tri = dalloc<float4>(tri.size()); upload(d_cell_tri, tri.data(), tri.size());
+      std::vector<float4> fpk(5 * (size_t)m.nf);
+      for (int f = 0; f < m.nf; ++f)
+        tdev::pack_face(reinterpret_cast<const float*>(uv.data()), xyz.data(), m.faces.data(), f, &fpk[5 * (size_t)f]);
+      d_face_pack = dalloc<float4>(fpk.size()); upload(d_face_pack, fpk.data(), fpk.size());
       const std::vector<double> cdf = face_area_cdf(m);
       d_cdf = dalloc<double>(cdf.size()); upload(d_cdf, cdf.data(), cdf.size());
       d_row_ptr = dalloc<int32_t>(m.row_ptr.size()); upload(d_row_ptr, m.row_ptr.data(), m.row_ptr.size());
@@ -189,6 +194,7 @@ struct tissue_mesh {
     tdev::MeshDev v;
     v.nv = m.nv; v.nf = m.nf; v.grid = m.grid; v.pad = 0;
     v.uv = d_uv; v.xyz = d_xyz; v.faces = d_faces; v.cell_start = d_cell_start; v.cell_tri = d_cell_tri;
+    v.face_pack = d_face_pack;
     v.row_ptr = d_row_ptr; v.nbr = d_nbr;
     return v;
   }

@@ -30,6 +30,7 @@ struct MeshDev {
   const int32_t* faces;    // [nf*3]
   const int32_t* cell_start;
   const float4* cell_tri;  // [2 * n_cell_faces]: candidate faces of each cell with their UV corners
+  const float4* face_pack; // [5 * nf]: each face's corner ids, UV and 3-D coordinates (pack_face)
   const int32_t* row_ptr;  // [nv+1]
   const int2* nbr;         // (vertex, float bits of the geodesic distance), rows ascending in distance
 };
@@ -374,6 +375,22 @@ inline void pack_cell_triangle(const float* uv, const int32_t* faces, int32_t f,
   out[1] = make_float4(uv[2 * c], uv[2 * c + 1], fbits, 0.0f);
 }
 
+// Face record read once the face is chosen: corner ids a, b, c (faces[3f ..]) with their UV and
+// 3-D coordinates, (ua, ub), (uc, a, b), (A, c), (B, 0), (C, 0), ids as float bits; one load level
+// instead of the face's ids and then its corners.
+inline void pack_face(const float* uv, const float* xyz, const int32_t* faces, int32_t f, float4* out) {
+  const int32_t a = faces[3 * f], b = faces[3 * f + 1], c = faces[3 * f + 2];
+  float ab, bb, cb;
+  __builtin_memcpy(&ab, &a, sizeof a);
+  __builtin_memcpy(&bb, &b, sizeof b);
+  __builtin_memcpy(&cb, &c, sizeof c);
+  out[0] = make_float4(uv[2 * a], uv[2 * a + 1], uv[2 * b], uv[2 * b + 1]);
+  out[1] = make_float4(uv[2 * c], uv[2 * c + 1], ab, bb);
+  out[2] = make_float4(xyz[3 * a], xyz[3 * a + 1], xyz[3 * a + 2], cb);
+  out[3] = make_float4(xyz[3 * b], xyz[3 * b + 1], xyz[3 * b + 2], 0.0f);
+  out[4] = make_float4(xyz[3 * c], xyz[3 * c + 1], xyz[3 * c + 2], 0.0f);
+}
+
 __host__ __device__ inline int float_bits(float x) {
   int b;
   __builtin_memcpy(&b, &x, sizeof b);
@@ -525,8 +542,13 @@ __host__ __device__ inline Located locate(const MeshDev& m, double px, double py
     best_f = bf;
   }
   if (best_f == 0x7fffffff) best_f = 0;  // non-finite point: any valid face keeps indices in range
-  const int ia = m.faces[3 * best_f], ib = m.faces[3 * best_f + 1], ic = m.faces[3 * best_f + 2];
-  const float2 ua = m.uv[ia], ub = m.uv[ib], uc = m.uv[ic];
+  const float4* fp = m.face_pack + 5 * (size_t)best_f;
+  const float4 p0 = fp[0], p1 = fp[1], p2 = fp[2], p3 = fp[3], p4 = fp[4];
+  const int ia = float_bits(p1.z), ib = float_bits(p1.w), ic = float_bits(p2.w);
+  const float2 ua = make_float2(p0.x, p0.y), ub = make_float2(p0.z, p0.w), uc = make_float2(p1.x, p1.y);
+  const float Ax = p2.x, Ay = p2.y, Az = p2.z;
+  const float Bx = p3.x, By = p3.y, Bz = p3.z;
+  const float Cx = p4.x, Cy = p4.y, Cz = p4.z;
   // FP32 blend and nearest-vertex choice, kept when the choice is clear
   {
     float wa = sqrtf(sqr(qx - ua.x) + sqr(qy - ua.y));
@@ -534,9 +556,6 @@ __host__ __device__ inline Located locate(const MeshDev& m, double px, double py
     float wc = sqrtf(sqr(qx - uc.x) + sqr(qy - uc.y));
     const float sw = wa + wb + wc;
     wa /= sw; wb /= sw; wc /= sw;
-    const float Ax = m.xyz[3 * ia], Ay = m.xyz[3 * ia + 1], Az = m.xyz[3 * ia + 2];
-    const float Bx = m.xyz[3 * ib], By = m.xyz[3 * ib + 1], Bz = m.xyz[3 * ib + 2];
-    const float Cx = m.xyz[3 * ic], Cy = m.xyz[3 * ic + 1], Cz = m.xyz[3 * ic + 2];
     const float X = wa * Ax + wb * Bx + wc * Cx, Y = wa * Ay + wb * By + wc * Cy, Z = wa * Az + wb * Bz + wc * Cz;
     const float da = sqrtf(sqr(X - Ax) + sqr(Y - Ay) + sqr(Z - Az));
     const float db = sqrtf(sqr(X - Bx) + sqr(Y - By) + sqr(Z - Bz));
@@ -552,9 +571,7 @@ __host__ __device__ inline Located locate(const MeshDev& m, double px, double py
     }
   }
   r.fp64 |= 2;
-  const double Ax = m.xyz[3 * ia], Ay = m.xyz[3 * ia + 1], Az = m.xyz[3 * ia + 2];
-  const double Bx = m.xyz[3 * ib], By = m.xyz[3 * ib + 1], Bz = m.xyz[3 * ib + 2];
-  const double Cx = m.xyz[3 * ic], Cy = m.xyz[3 * ic + 1], Cz = m.xyz[3 * ic + 2];
+  const double Ax64 = Ax, Ay64 = Ay, Az64 = Az, Bx64 = Bx, By64 = By, Bz64 = Bz, Cx64 = Cx, Cy64 = Cy, Cz64 = Cz;
   double w_a = sqrt(sqr(px - (double)ua.x) + sqr(py - (double)ua.y));
   double w_b = sqrt(sqr(px - (double)ub.x) + sqr(py - (double)ub.y));
   double w_c = sqrt(sqr(px - (double)uc.x) + sqr(py - (double)uc.y));
@@ -562,12 +579,12 @@ __host__ __device__ inline Located locate(const MeshDev& m, double px, double py
   w_a /= sum_weights;
   w_b /= sum_weights;
   w_c /= sum_weights;
-  r.X = w_a * Ax + w_b * Bx + w_c * Cx;
-  r.Y = w_a * Ay + w_b * By + w_c * Cy;
-  r.Z = w_a * Az + w_b * Bz + w_c * Cz;
-  const double da = sqrt(sqr(r.X - Ax) + sqr(r.Y - Ay) + sqr(r.Z - Az));
-  const double db = sqrt(sqr(r.X - Bx) + sqr(r.Y - By) + sqr(r.Z - Bz));
-  const double dc = sqrt(sqr(r.X - Cx) + sqr(r.Y - Cy) + sqr(r.Z - Cz));
+  r.X = w_a * Ax64 + w_b * Bx64 + w_c * Cx64;
+  r.Y = w_a * Ay64 + w_b * By64 + w_c * Cy64;
+  r.Z = w_a * Az64 + w_b * Bz64 + w_c * Cz64;
+  const double da = sqrt(sqr(r.X - Ax64) + sqr(r.Y - Ay64) + sqr(r.Z - Az64));
+  const double db = sqrt(sqr(r.X - Bx64) + sqr(r.Y - By64) + sqr(r.Z - Bz64));
+  const double dc = sqrt(sqr(r.X - Cx64) + sqr(r.Y - Cy64) + sqr(r.Z - Cz64));
   double mn = da;
   if (db < mn) mn = db;
   if (dc < mn) mn = dc;

@@ -79,6 +79,9 @@ void hc_locate_many(int n, const double* pts, int nv, int nf, const float* uv, c
   std::vector<float4> tri(2 * (size_t)cell_start[grid * grid]);
   for (int q = 0; q < cell_start[grid * grid]; ++q) tdev::pack_cell_triangle(uv, faces, cell_faces[q], &tri[2 * (size_t)q]);
   m.cell_tri = tri.data();
+  std::vector<float4> fpk(5 * (size_t)nf);
+  for (int f = 0; f < nf; ++f) tdev::pack_face(uv, xyz, faces, f, &fpk[5 * (size_t)f]);
+  m.face_pack = fpk.data();
   for (int i = 0; i < n; ++i) {
     const tdev::Located L = tdev::locate<1>(m, pts[2 * i], pts[2 * i + 1], 0);
     face[i] = L.face;
