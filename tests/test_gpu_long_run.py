@@ -1,4 +1,4 @@
-"""Long runs are judged by order-parameter statistics. With the reference's d == 0 rule a particle
+"""Long runs are compared through order-parameter statistics. With the reference's d == 0 rule a particle
 sharing a vertex with another is thrown across the chart, so two trajectories from the same inputs
 part within a few steps: storing the FP64 oracle's own state in FP32 moves particles by more than
 1e-5 after two steps and decorrelates the whole system after about six (test_oracle.py pins this
