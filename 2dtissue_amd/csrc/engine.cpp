@@ -353,15 +353,17 @@ void Engine::ensure_tiles(const Shape& s) {
 
 void Engine::stage_noise(const float* h, int rows) {
   if (!h || rows < 1) throw ApiError(TISSUE_E_INVALID_ARG, "bad noise rows");
-  if (rows > noise_rows) {
+  if (rows > noise_cap) {
     HIP_OK(hipStreamSynchronize(stream));
     drop_graphs();
     dfree(noise);
+    noise_rows = 0;
     noise = dalloc<float>((size_t)rows * ntot);
-    noise_rows = rows;
+    noise_cap = rows;
   }
   HIP_OK(hipMemcpyAsync(noise, h, (size_t)rows * ntot * sizeof(float), hipMemcpyHostToDevice, stream));
   HIP_OK(hipStreamSynchronize(stream));
+  noise_rows = rows;  // rows past this count hold an earlier staging: out of range from now on
 }
 
 void Engine::gen_noise(const uint64_t* seeds, const double* stds, int rows) {

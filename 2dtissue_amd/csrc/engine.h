@@ -122,7 +122,8 @@ struct Engine {
   double* phi = nullptr;
   size_t phi_cap = 0;
   float* noise = nullptr;  // staged rows
-  int noise_rows = 0;
+  int noise_rows = 0;      // rows of the last staging (the valid ones)
+  int noise_cap = 0;       // rows the buffer holds
   float* call_noise = nullptr;
   size_t call_noise_cap = 0;
 

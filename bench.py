@@ -20,7 +20,11 @@ Also reported:
     (phi_T - 0.8)^2 + 0.1 mean |rdot - u|^2, particle-steps/s of forward and reverse together.
   large: perturbed sphere (F = 200,000), 100,000 particles with sigma 0.01 for 500 steps, and
     64 x 2,000 particles forward + adjoint together over 200 steps.
-  cpu_baseline: the FP64 oracle's dense reference algorithm on a bounded sample, single thread.
+  cpu_baseline: the FP64 oracle's dense reference algorithm on a bounded sample, single thread;
+    cpu_baseline_all_cores: the same on every core this process may use (affinity mask, capped
+    by the cgroup quota), with the host's `lscpu` model name.
+A secondary measurement that fails is reported as {"error": ...} in its own field; the headline
+line is printed regardless.
 Roofline: algorithmic bytes per particle-step (64 B: read x, y, theta, vertex id, eta; write x,
 y, theta, vx, vy, r3d, vertex id, face, neighbour count) over the kernel's HIP-event time, against
 8 TB/s.
@@ -114,6 +118,15 @@ def valu_axis(ps: float, pairs_per_particle: float, faces_per_locate: float, iss
 
 
 def cpu_model() -> str:
+    """The host CPU's model name as `lscpu` reports it (falls back to /proc/cpuinfo)."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.strip().lower().startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
@@ -122,6 +135,22 @@ def cpu_model() -> str:
     except OSError:
         pass
     return "unknown"
+
+
+def host_cores() -> dict:
+    """Cores this process may run on: `nproc` (the affinity mask), os.cpu_count() (every core of
+    the machine) and the cgroup CPU quota; `usable` is the affinity count capped by the quota."""
+    nproc = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+            if q != "max":
+                quota = float(q) / float(period)
+    except (OSError, ValueError):
+        pass
+    usable = nproc if quota is None else max(1, min(nproc, int(quota + 0.5)))
+    return dict(nproc=nproc, cpu_count=os.cpu_count(), cgroup_quota_cpus=quota, usable=usable)
 
 
 def warm_graph(run, steps: int, row: int) -> None:
@@ -135,24 +164,30 @@ def warm_graph(run, steps: int, row: int) -> None:
         run(extra, row)
 
 
-def cpu_baseline(tables, x, y, th, vid, noise, steps, threads=1):
+def cpu_baseline(T, tables, s0, steps, threads=1, min_seconds=0.0):
     """The oracle's FP64 dense reference algorithm (N x N matrices, brute-force remap over all
-    faces, restart tiling loop), timed on this host for a bounded number of steps."""
+    faces, restart tiling loop), timed on this host over a bounded sample: `steps` steps from the
+    plain workload's initial state with its own noise rows (the workload's noise seed), continued
+    up to 4 x `steps` while less than `min_seconds` have passed."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     O.set_threads(threads)
     om = O.Mesh(tables)
     w = WORKLOAD
     p = dict(sigma=w["sigma"], v0=w["v0"], k=w["k"], mu=w["mu"], tau=w["tau"], dt=w["dt"])
+    noise = T.gen_noise(w["noise_seed"], w["noise_std"], 4 * steps, w["n"]).astype(np.float64)
+    x, y, th, vid = (s0.x.astype(np.float64), s0.y.astype(np.float64), s0.theta.astype(np.float64), s0.vid)
     t0 = time.perf_counter()
-    for t in range(steps):
-        st, s, _ = O.step(om, p, x, y, th, vid, noise[t].astype(np.float64), remap_mode=0)
+    done = 0
+    while done < len(noise) and (done < steps or time.perf_counter() - t0 < min_seconds):
+        st, s, _ = O.step(om, p, x, y, th, vid, noise[done], remap_mode=0)
         x, y, th, vid = s["x"], s["y"], s["theta"], s["vid"]
+        done += 1
     dt = time.perf_counter() - t0
-    return dict(value=w["n"] * steps / dt, unit="particle-steps/s", cores=threads, kind="port",
-                cpu_model=cpu_model(),
-                sample=f"{steps} steps of the {w['n']}-particle workload (FP64, dense N^2 forces/alignment, "
-                       f"brute-force remap over all {tables['nf']} faces), {dt:.1f} s wall")
+    return dict(value=w["n"] * done / dt, unit="particle-steps/s", cores=threads, kind="port",
+                cpu_model=cpu_model(), host=host_cores(),
+                sample=f"{done} steps of the {w['n']}-particle workload (FP64, dense N^2 forces/alignment, "
+                       f"brute-force remap over all {tables['nf']} faces) on {threads} thread(s), {dt:.1f} s wall")
 
 
 def plain(T, mesh, mode, lanes, steps, warmup):
@@ -177,7 +212,7 @@ def plain(T, mesh, mode, lanes, steps, warmup):
     res = dict(code=code, elapsed=elapsed, kernel_ms=sim.last_step_ms(), flags=sim.flags()[0],
                phi_polar_last=float(sim.order()[1][-1]), launch=sim.launch(), pairs=pairs)
     sim.close()
-    return res, s0, noise_h
+    return res, s0
 
 
 def batched(T, mesh, steps, n_replicas=256, plain_sample=16):
@@ -363,6 +398,16 @@ def copy_calibration(mib: int, reps: int = 5) -> dict:
     return dict(bytes_per_copy=4 * n, copies=reps)
 
 
+def leg(out: dict, key: str, fn, *args, **kw) -> None:
+    """Runs one secondary measurement into out[key]; a failure is recorded there as
+    {"error": ...} and the other legs, and the JSON line, still go ahead."""
+    try:
+        out[key] = fn(*args, **kw)
+    except Exception as e:  # noqa: BLE001 -- every failure of a leg is reported, none is fatal
+        log(f"{key}: FAILED: {type(e).__name__}: {e}")
+        out[key] = {"error": f"{type(e).__name__}: {e}"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -372,6 +417,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--batched-steps", type=int, default=500)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-resident", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-large", action="store_true")
@@ -392,9 +438,7 @@ def main():
     info = mesh.info()
 
     log(f"plain, default mode: warmup {args.warmup}, timing {args.steps} steps")
-    main_res, s0, noise_h = plain(T, mesh, T.MODE_AUTO, args.lanes, args.steps, args.warmup)
-    log(f"plain, persistent kernel: timing {args.steps} steps")
-    res_res, _, _ = plain(T, mesh, T.MODE_RESIDENT, args.lanes, args.steps, args.warmup)
+    main_res, s0 = plain(T, mesh, T.MODE_AUTO, args.lanes, args.steps, args.warmup)
 
     n = w["n"]
     ps = n * args.steps / main_res["elapsed"]
@@ -406,6 +450,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": main_res["elapsed"] * 1e3 / args.steps,
+        "gpu_ms_per_step": main_res["kernel_ms"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -416,34 +461,41 @@ def main():
                         "sigma=0.05, v0=1, mu=1, k=10, tau=1, dt=0.02, noise N(0,0.1^2); one launch per step",
             "particles": n, "faces": tables["nf"], "uv_vertices": tables["nv"], "status": main_res["code"],
             "flags": main_res["flags"],
+            "timing": "value and ms_per_step: host wall clock around one call of `steps` steps between device "
+                      "synchronisations; gpu_ms_per_step: HIP events on the simulation's stream around the "
+                      "same steps",
         },
         "roofline": roofline(n, main_res["kernel_ms"] / 1e3 / args.steps, step_kernels(n, tables["nv"]),
                              workload="plain", launch=main_res["launch"]),
         "valu": valu_axis(ps, main_res["pairs"], info["n_cell_faces"] / info["grid"] ** 2,
                           (measured("plain", main_res["launch"]) or ({},))[0].get("valu_issue_frac")),
         "launch": main_res["launch"],
-        "plain_resident_kernel": {
-            "value": n * args.steps / res_res["elapsed"], "unit": "particle-steps/s",
-            "ms_per_step": res_res["elapsed"] * 1e3 / args.steps, "status": res_res["code"],
-            "roofline": roofline(n, res_res["kernel_ms"] / 1e3 / args.steps, "k_resident (one step)"),
-        },
     }
+
+    def resident():
+        r, _ = plain(T, mesh, T.MODE_RESIDENT, args.lanes, args.steps, args.warmup)
+        return {"value": n * args.steps / r["elapsed"], "unit": "particle-steps/s",
+                "ms_per_step": r["elapsed"] * 1e3 / args.steps, "status": r["code"],
+                "roofline": roofline(n, r["kernel_ms"] / 1e3 / args.steps, "k_resident (one step)")}
+
+    if not args.no_resident:
+        log(f"plain, persistent kernel: timing {args.steps} steps")
+        leg(out, "plain_resident_kernel", resident)
     if not args.no_batched:
-        out["batched"] = batched(T, mesh, args.batched_steps)
+        leg(out, "batched", batched, T, mesh, args.batched_steps)
     if not args.no_adjoint:
         log("forward + adjoint: 1000 particles, 200 steps")
-        out["adjoint"] = adjoint(T, mesh)
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+        leg(out, "adjoint", adjoint, T, mesh)
+    cores = host_cores()
+    threads = cores["usable"]
     if not args.no_large:
-        out["large"] = large(T, cpu_threads=0 if args.no_cpu else threads)
+        leg(out, "large", large, T, cpu_threads=0 if args.no_cpu else threads)
     if not args.no_cpu:
-        log(f"CPU baseline: {args.cpu_steps} oracle steps")
-        out["cpu_baseline"] = cpu_baseline(tables, s0.x.astype(np.float64), s0.y.astype(np.float64),
-                                           s0.theta.astype(np.float64), s0.vid, noise_h, args.cpu_steps)
-        log(f"CPU baseline on {threads} threads: {args.cpu_steps} oracle steps")
-        out["cpu_baseline_all_cores"] = cpu_baseline(tables, s0.x.astype(np.float64), s0.y.astype(np.float64),
-                                                     s0.theta.astype(np.float64), s0.vid, noise_h, args.cpu_steps,
-                                                     threads=threads)
+        log(f"CPU baseline: {args.cpu_steps} oracle steps on 1 thread")
+        leg(out, "cpu_baseline", cpu_baseline, T, tables, s0, args.cpu_steps)
+        log(f"CPU baseline on all {threads} usable cores ({cores})")
+        leg(out, "cpu_baseline_all_cores", cpu_baseline, T, tables, s0, args.cpu_steps, threads=threads,
+            min_seconds=10.0)
     if calib:
         out["copy_calibration"] = calib
     print(json.dumps(out), flush=True)

@@ -5,6 +5,14 @@ Tolerances (FP32 GPU vs FP64 oracle on identical FP32 inputs):
   headings: compared modulo 2 pi, same bound
   neighbour sets, neighbour counts: exact
   point location (face, vertex id): exact, against the oracle's remap of the GPU's stored position
+  order parameters (LinearAlgebra.cpp:90-105 as called at 2DTissue.cpp:242, z := 0):
+    phi_polar = |sum (cos theta, sin theta)| / N: |gpu - ref| <= 1e-5 * ref
+    phi_ref = |sum sign(x vy - y vx)| / N: exact. The GPU's value must equal the host's FP64
+      evaluation of the same formula on the GPU's own FP32 outputs bit for bit, and each particle's
+      sign must equal the oracle's, except for particles whose |x vy - y vx| lies within the
+      rounding bound of the compared quantities (the TOL bounds on x, y, vx, vy propagated
+      through the cross product); those are counted and reported, and N |phi_gpu - phi_ref| may
+      differ by at most twice their number.
 """
 from __future__ import annotations
 
@@ -51,10 +59,45 @@ def close(gpu, ref, tol=TOL):
     return np.abs(gpu - ref) <= tol * np.maximum(1.0, np.abs(ref))
 
 
-def compare_step(O, om, p, s_in, s_out, out, nbrs_gpu, eta, sparse=False):
+def last_phi(sim):
+    """(phi_ref, phi_polar) of the last step a Sim ran (its order-parameter history's last entry)."""
+    a, b = sim.order()
+    return float(a[-1]), float(b[-1])
+
+
+def cross_sign_bound(x, y, vx, vy, tol=TOL):
+    """Per particle: how far x vy - y vx can move when each of x, y, vx, vy moves by the parity
+    tolerance tol * max(1, |value|) (first-order propagation, doubled for the second-order term)."""
+    ax, ay, avx, avy = (np.abs(np.asarray(a, np.float64)) for a in (x, y, vx, vy))
+    ex, ey, evx, evy = (tol * np.maximum(1.0, a) for a in (ax, ay, avx, avy))
+    return 2.0 * (ex * avy + ax * evy + ey * avx + ay * evx + ex * evy + ey * evx)
+
+
+def compare_phi(s_out, out, ref_s, ref_o, phi_gpu):
+    """phi_ref and phi_polar of one GPU step against the oracle (tolerances in the module doc).
+    Returns (failure counts, number of particles exempt from the sign comparison)."""
+    n = len(s_out.x)
+    x, y = s_out.x.astype(np.float64), s_out.y.astype(np.float64)
+    vx, vy = out.vx.astype(np.float64), out.vy.astype(np.float64)
+    cz_gpu = x * vy - y * vx  # the kernel's FP64 evaluation on its FP32 outputs (no contraction)
+    sg_gpu = np.sign(cz_gpu).astype(np.int64)
+    cz_ref = ref_s["x"] * ref_o["vy"] - ref_s["y"] * ref_o["vx"]
+    sg_ref = np.sign(cz_ref).astype(np.int64)
+    exempt = np.abs(cz_ref) <= cross_sign_bound(ref_s["x"], ref_s["y"], ref_o["vx"], ref_o["vy"])
+    fails = {}
+    fails["phi_ref_sign"] = int(((sg_gpu != sg_ref) & ~exempt).sum())
+    fails["phi_ref_self"] = int(phi_gpu[0] != (1.0 / n) * abs(float(sg_gpu.sum())))
+    fails["phi_ref"] = int(abs(n * phi_gpu[0] - n * ref_o["phi_ref"]) > 2 * int(exempt.sum()) + 1e-9)
+    fails["phi_polar"] = int(abs(phi_gpu[1] - ref_o["phi_polar"]) > TOL * ref_o["phi_polar"])
+    return fails, int(exempt.sum())
+
+
+def compare_step(O, om, p, s_in, s_out, out, nbrs_gpu, eta, sparse=False, phi_gpu=None):
     """Compare one GPU step (s_in -> s_out, outputs) with the oracle's step from the same inputs.
-    sparse selects the oracle's O(N) step, bitwise equal to the dense one. Returns a dict of
-    failure counts (all zero when the step matches)."""
+    sparse selects the oracle's O(N) step, bitwise equal to the dense one. phi_gpu: the step's
+    (phi_ref, phi_polar) from the GPU (compared when given). Returns a dict of failure counts (all
+    zero when the step matches); the count of particles exempt from the phi_ref sign comparison is
+    returned in the second value's "phi_exempt"."""
     if sparse:
         st, ref_s, ref_o = O.step_sparse(om, p, s_in.x, s_in.y, s_in.theta, s_in.vid, eta)
     else:
@@ -73,4 +116,8 @@ def compare_step(O, om, p, s_in, s_out, out, nbrs_gpu, eta, sparse=False):
     fails["vid"] = int((vid != s_out.vid).sum())
     fails["face"] = int((face != out.face).sum())
     fails["r3d"] = int((~close(out.r3d, r3d)).any(axis=1).sum())
+    if phi_gpu is not None:
+        f_phi, exempt = compare_phi(s_out, out, ref_s, ref_o, phi_gpu)
+        fails.update(f_phi)
+        ref_o = dict(ref_o, phi_exempt=exempt)
     return fails, ref_o

@@ -58,7 +58,7 @@ def test_resident_teacher_forced_c1(T, O):
         nb = sim.neighbor_lists()
         assert sim.step(1, noise[t]) == T.OK
         fails, _ = P.compare_step(O, om, p, s_in, sim.get_state(), sim.get_outputs(), nb,
-                                  noise[t].astype(np.float64))
+                                  noise[t].astype(np.float64), phi_gpu=P.last_phi(sim))
         for k, v in fails.items():
             total[k] = total.get(k, 0) + v
     assert all(v == 0 for v in total.values()), total

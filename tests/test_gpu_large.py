@@ -30,9 +30,9 @@ def test_large_surface_teacher_forced(T, O, first):
         s_in = sim.get_state()
         nb = sim.neighbor_lists()
         assert sim.step(1, noise[t]) == T.OK
-        fails, _ = P.compare_step(O, om, p, s_in, sim.get_state(), sim.get_outputs(), nb,
-                                  noise[t].astype(np.float64), sparse=True)
-        print(t, fails, flush=True)
+        fails, ref = P.compare_step(O, om, p, s_in, sim.get_state(), sim.get_outputs(), nb,
+                                    noise[t].astype(np.float64), sparse=True, phi_gpu=P.last_phi(sim))
+        print(t, fails, f"phi_ref sign exempt: {ref['phi_exempt']}", flush=True)
         for k, v in fails.items():
             total[k] = total.get(k, 0) + v
     assert all(v == 0 for v in total.values()), total

@@ -19,17 +19,23 @@ def run_teacher_forced(T, O, cfg, steps, lanes=0, mode=0, sparse=False):
     sim.init_random(cfg["seed"])
     noise = T.gen_noise(cfg["noise_seed"], 0.1, steps, cfg["n"])
     total = {}
+    exempt = 0
     for t in range(steps):
         s_in = sim.get_state()
         nb = sim.neighbor_lists()
         assert sim.step(1, noise[t]) == T.OK
         s_out = sim.get_state()
         out = sim.get_outputs()
-        fails, _ = P.compare_step(O, om, p, s_in, s_out, out, nb, noise[t].astype(np.float64), sparse)
+        fails, ref = P.compare_step(O, om, p, s_in, s_out, out, nb, noise[t].astype(np.float64), sparse,
+                                    phi_gpu=P.last_phi(sim))
+        exempt += ref["phi_exempt"]
         for k, v in fails.items():
             total[k] = total.get(k, 0) + v
         if any(fails.values()):
             print(f"step {t}: {fails}", flush=True)
+    print(f"{cfg['n']} particles x {steps} steps: {exempt} particle-steps exempt from the phi_ref sign "
+          f"comparison (|x vy - y vx| within its rounding bound)", flush=True)
+    sim.close()
     return total
 
 
@@ -100,3 +106,20 @@ def test_gpu_initialisation_matches_host(T):
         assert np.array_equal(bs.theta[o:o + nr], th), r
         o += nr
     b.close()
+
+
+def test_restaged_fewer_rows_are_the_valid_range(T):
+    """Staging 40 noise rows and then 10 leaves only the 10 in range: a run that asks for rows the
+    second staging did not write is rejected instead of reading the first staging's leftovers."""
+    cfg = P.C1
+    m, _ = P.mesh_and_tables(cfg["freq"], cfg["sigma"])
+    s = T.Sim(m, T.params(**dict(P.PHYS, sigma=cfg["sigma"])), cfg["n"])
+    s.init_random(cfg["seed"])
+    s.stage_noise(T.gen_noise(1, 0.1, 40, cfg["n"]))
+    assert s.step_staged(5, 20) == T.OK
+    s.stage_noise(T.gen_noise(2, 0.1, 10, cfg["n"]))
+    with pytest.raises(T.TissueError) as e:
+        s.step_staged(5, 20)
+    assert e.value.code == T.E_INVALID_ARG
+    assert s.step_staged(5, 5) == T.OK
+    s.close()

@@ -195,6 +195,78 @@ def test_remap_bucketed_equals_brute_force(O, mesh1):
     assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2]) and np.array_equal(a[0], b[0])
 
 
+def remap_probe_points(tb, rng, n_random, vertex_sample=None, face_sample=None, corner_sample=None):
+    """Points that stress a face-bucket grid: uniform random ones, UV vertices, the midpoints of
+    all three edges of faces, the square's corners, and points on the grid's cell boundaries
+    (exactly on a grid line and one ulp to either side; cell corners too)."""
+    uv, f, G = tb["uv"], tb["faces"], tb["grid"]
+    vs = np.arange(len(uv)) if vertex_sample is None else rng.choice(len(uv), vertex_sample, replace=False)
+    fs = np.arange(len(f)) if face_sample is None else rng.choice(len(f), face_sample, replace=False)
+    xs = [rng.random(n_random), uv[vs, 0], [0.0, 1.0, 0.0, 1.0]]
+    ys = [rng.random(n_random), uv[vs, 1], [0.0, 0.0, 1.0, 1.0]]
+    for a, b in ((0, 1), (1, 2), (2, 0)):
+        xs.append(0.5 * (uv[f[fs, a], 0] + uv[f[fs, b], 0]))
+        ys.append(0.5 * (uv[f[fs, a], 1] + uv[f[fs, b], 1]))
+    lines = np.arange(G + 1) / G
+    k = max(1, n_random // (4 * (G + 1)))
+    for d in (0.0, -1.0, 1.0):  # on the line, one ulp below, one ulp above
+        on = np.clip(np.nextafter(lines, lines + d) if d else lines, 0.0, 1.0)
+        xs += [np.repeat(on, k), rng.random(len(on) * k)]
+        ys += [rng.random(len(on) * k), np.repeat(on, k)]
+    cx, cy = (a.ravel() for a in np.meshgrid(lines, lines))
+    cs = np.arange(len(cx)) if corner_sample is None else rng.choice(len(cx), corner_sample, replace=False)
+    xs.append(cx[cs]); ys.append(cy[cs])
+    return np.concatenate(xs), np.concatenate(ys)
+
+
+@pytest.mark.parametrize("surface", ["ellipsoid22", "ellipsoid32", "sphere100"])
+def test_remap_bucketed_equals_brute_force_parity_meshes(O, surface):
+    """The face-bucket remap (the GPU's and the parity tests' remap) equals the brute-force remap
+    over every face (CellHelper.cpp:104-117: nearest face, lowest index on ties) on the meshes the
+    parity tests run on: the ellipsoids of the 100- and 2,000-particle workloads and, on a sample of
+    over 20,000 points, the frequency-100 perturbed sphere of the 100,000-particle workload."""
+    import parity as P
+    O.set_threads(os.cpu_count() or 1)
+    rng = np.random.default_rng(17)
+    if surface == "ellipsoid22":
+        _, tb = P.mesh_and_tables(22, 0.1)
+        x, y = remap_probe_points(tb, rng, 5000)
+    elif surface == "ellipsoid32":
+        _, tb = P.mesh_and_tables(32, 0.05)
+        x, y = remap_probe_points(tb, rng, 5000)
+    else:
+        _, tb = P.sphere_and_tables(100, 0.2, 0.01)
+        x, y = remap_probe_points(tb, rng, 12000, vertex_sample=4000, face_sample=2000, corner_sample=4000)
+    assert len(x) >= 20000
+    om = O.Mesh(tb)
+    a = O.remap(om, x, y, 0)
+    b = O.remap(om, x, y, 1)
+    bad = np.nonzero(a[2] != b[2])[0]
+    assert len(bad) == 0, (len(bad), x[bad[:5]], y[bad[:5]], a[2][bad[:5]], b[2][bad[:5]])
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[0], b[0])
+
+
+def test_remap_bucket_check_catches_a_missing_face(O):
+    """Control for the comparison above: a face bucket that lacks one face that overlaps its cell
+    (planted in a copy of the tables) makes the bucketed remap differ from brute force at that
+    face's centroid."""
+    import parity as P
+    _, tb = P.mesh_and_tables(22, 0.1)
+    uv, f, G = tb["uv"], tb["faces"], tb["grid"]
+    fi = 1234
+    c = uv[f[fi]].mean(axis=0)
+    cell = int(min(np.floor(c[1] * G), G - 1)) * G + int(min(np.floor(c[0] * G), G - 1))
+    s0, s1 = tb["cell_start"][cell], tb["cell_start"][cell + 1]
+    lst = tb["cell_faces"]
+    k = s0 + int(np.nonzero(lst[s0:s1] == fi)[0][0])
+    bad = dict(tb, cell_faces=np.concatenate([lst[:k], lst[k + 1:]]),
+               cell_start=np.where(np.arange(len(tb["cell_start"])) > cell, tb["cell_start"] - 1, tb["cell_start"]))
+    om = O.Mesh(bad)
+    a = O.remap(om, [c[0]], [c[1]], 0)
+    b = O.remap(om, [c[0]], [c[1]], 1)
+    assert a[2][0] == fi and b[2][0] != fi
+
+
 def test_remap_closed_form(O, mesh1):
     """Face = the UV triangle containing the point; r3d = distance-weighted blend of its 3-D
     corners (weights = normalised distances, CellHelper.cpp:133-141); vertex = corner nearest r3d."""
