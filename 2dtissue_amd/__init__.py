@@ -34,6 +34,7 @@ E_MESH = 7
 FLAG_LOST_PARTICLE = 1
 FLAG_NONFINITE = 2
 FLAG_TILING = 4
+FLAG_BAD_INPUT = 8
 
 _STATUS_NAMES = {1: "E_INVALID_ARG", 2: "E_LOST_PARTICLE", 3: "E_NONFINITE", 4: "E_TILING_NO_CONVERGE",
                  5: "E_HIP", 6: "E_IO", 7: "E_MESH"}
@@ -166,6 +167,7 @@ SIGNATURES = [
     ("tissue_random_init_host", C.c_int32, [_P, C.c_uint64, C.c_int32, _F, _F, _F]),
     ("tissue_record_start", C.c_int32, [_P, C.c_char_p, C.c_int32]),
     ("tissue_record_stop", C.c_int32, [_P]),
+    ("tissue_step_noalloc", C.c_int32, [_P, C.POINTER(Params), C.c_int32] + [C.c_void_p] * 19),
 ]
 
 

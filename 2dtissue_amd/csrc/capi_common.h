@@ -191,6 +191,20 @@ struct tissue_mesh {
       d_nbr = dalloc<int2>(nbr.size()); upload(d_nbr, nbr.data(), nbr.size());
       dev_ready = true;
     }
+    return view();
+  }
+
+  // the device tables when they are already on GPU `dev` (no upload, no allocation)
+  tdev::MeshDev resident_view(int dev) {
+    std::lock_guard<std::mutex> lk(mu);
+    if (!tables) throw ApiError(TISSUE_E_INVALID_ARG, "mesh tables not built (tissue_mesh_build_tables)");
+    if (!dev_ready || device != dev)
+      throw ApiError(TISSUE_E_INVALID_ARG, "mesh tables are not on this GPU (a tissue_create on the mesh uploads them)");
+    return view();
+  }
+
+ private:
+  tdev::MeshDev view() const {
     tdev::MeshDev v;
     v.nv = m.nv; v.nf = m.nf; v.grid = m.grid; v.pad = 0;
     v.uv = d_uv; v.xyz = d_xyz; v.faces = d_faces; v.cell_start = d_cell_start; v.cell_tri = d_cell_tri;

@@ -64,13 +64,8 @@ Engine::Engine(tissue_mesh* m, int R_, const int* n_, const tissue_params* p_) {
     }
     if (tot > INT_MAX / 16) throw ApiError(TISSUE_E_INVALID_ARG, "too many particles");
     ntot = (int)tot;
-    // replicas of a few thousand particles, at most one per two vertices, leave their vertex
-    // buckets raw: the finalize pass would cost a kernel boundary per step, and the readers order
-    // the few particles of each bucket themselves (the same visiting order, so the same results).
-    // Denser or larger systems pile particles onto vertices (the exit snapping), and a crowded raw
-    // bucket costs each reader a scan of the overflow list. TISSUE_SORTED_BUCKETS = 1 / 0 forces
-    // the choice.
-    raw_buckets = nmax <= kRawBucketsMax && 2L * nmax <= md.nv;
+    // raw or sorted vertex buckets (raw_buckets_for); TISSUE_SORTED_BUCKETS = 1 / 0 forces the choice
+    raw_buckets = raw_buckets_for(nmax, md.nv);
     if (const char* e = std::getenv("TISSUE_SORTED_BUCKETS")) {
       if (e[0] == '1') raw_buckets = false;
       if (e[0] == '0') raw_buckets = true;
@@ -443,13 +438,10 @@ Shape Engine::resolve_for(Shape s) const {
     s.spl = s.spl > 0 ? s.spl : kDefaultSpl;
     s.xcd = 0;
   } else {
-    // measured on MI355X: a single system of a few thousand particles fills few CUs, so more lanes
-    // per particle shorten its critical path (2,000 particles: 16 lanes 25.2 us, 8 lanes 27.2 us);
-    // a single 100,000-particle system is fastest with 4 (350 us; 2 lanes 383 us), a large batch
-    // with 2 (autotuned)
-    const int big_lanes = R == 1 ? 4 : 2;
-    s.lanes = s.lanes > 0 ? s.lanes : (ntot <= 4096 ? 16 : (ntot <= 8192 ? 8 : (ntot <= 65536 ? 4 : big_lanes)));
-    s.block = s.block > 0 ? s.block : (ntot <= 4096 ? 128 : 256);
+    int lanes = 0, block = 0;
+    tiled_default_shape(R, ntot, &lanes, &block);
+    s.lanes = s.lanes > 0 ? s.lanes : lanes;
+    s.block = s.block > 0 ? s.block : block;
     s.spl = s.spl > 0 ? s.spl : kDefaultSpl;
     s.xcd = s.xcd >= 0 ? s.xcd : (R >= 2 * kXcds ? 1 : 0);
   }

@@ -91,9 +91,7 @@ struct Engine {
   std::vector<tissue_params> p;
   std::vector<ReplicaDesc> reps;
   bool has_state = false;
-  // vertex buckets left raw (no finalize pass; readers order each bucket), for replicas of up to
-  // kRawBucketsMax particles and at most one particle per two vertices
-  static constexpr int kRawBucketsMax = 8192;
+  // vertex buckets left raw (no finalize pass; readers order each bucket): raw_buckets_for
   bool raw_buckets = false;
 
   ReplicaDesc* d_reps = nullptr;

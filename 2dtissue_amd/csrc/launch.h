@@ -109,6 +109,25 @@ constexpr int kResidentMaxBlock = 512;
 // lanes per particle for the persistent kernel when not tuned: wider groups for smaller replicas
 inline int resident_default_lanes(int nmax) { return nmax <= 256 ? 8 : (nmax <= 1024 ? 4 : (nmax <= 2048 ? 2 : 1)); }
 
+// Default launch shape of the tiled step kernel (lanes per particle, threads per workgroup) for
+// R replicas of ntot particles in all. Measured on MI355X: a single system of a few thousand
+// particles fills few CUs, so more lanes per particle shorten its critical path (2,000 particles:
+// 16 lanes 25.2 us, 8 lanes 27.2 us); a single 100,000-particle system is fastest with 4 (350 us;
+// 2 lanes 383 us), a large batch with 2 (autotuned).
+inline void tiled_default_shape(int R, int ntot, int* lanes, int* block) {
+  const int big_lanes = R == 1 ? 4 : 2;
+  *lanes = ntot <= 4096 ? 16 : (ntot <= 8192 ? 8 : (ntot <= 65536 ? 4 : big_lanes));
+  *block = ntot <= 4096 ? 128 : 256;
+}
+
+// Replicas of a few thousand particles, at most one per two vertices, leave their vertex buckets
+// raw: the finalize pass would cost a kernel boundary per step, and the readers order the few
+// particles of each bucket themselves (the same visiting order, so the same results). Denser or
+// larger systems pile particles onto vertices (the exit snapping), and a crowded raw bucket costs
+// each reader a scan of the overflow list.
+constexpr int kRawBucketsMax = 8192;
+inline bool raw_buckets_for(int nmax, int nv) { return nmax <= kRawBucketsMax && 2L * nmax <= nv; }
+
 // LDS bytes of one workgroup of the persistent kernel for a mesh with nv vertices and replicas of
 // at most nmax particles; resident_max_particles is the largest nmax that fits one CU
 size_t resident_lds_bytes(int nv, int nmax);

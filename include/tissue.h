@@ -39,6 +39,7 @@ extern "C" {
 #define TISSUE_FLAG_LOST_PARTICLE 1u
 #define TISSUE_FLAG_NONFINITE 2u
 #define TISSUE_FLAG_TILING 4u
+#define TISSUE_FLAG_BAD_INPUT 8u      /* tissue_step_noalloc: a vertex id outside the mesh was given      */
 
 typedef struct tissue_mesh tissue_mesh;
 typedef struct tissue_sim tissue_sim;
@@ -232,6 +233,32 @@ int32_t tissue_record_stop(tissue_sim* s);
  * those reached at the bound). Host arrays; runs synchronously on the current device. */
 int32_t tissue_seam_wrap(int32_t n, const double* start_xy, double* end_xy, double* dtheta, int32_t* quarter,
                          int32_t max_crossings, int32_t lanes);
+
+/* ---- one step without allocation ----
+ * No counterpart in the reference. For callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc. One step of one system
+ * of n particles, as one tissue_step of a simulation on the same mesh with the same parameters
+ * and state (default launch shape of the per-step kernel), bit for bit. Every pointer except
+ * mesh and p is device memory on the current GPU: the state (x_in, y_in, theta_in, vid_in [n]),
+ * the noise eta [n] (NULL: zero), and the outputs (the new state x_out, y_out, theta_out,
+ * vid_out [n]; r_dot vx, vy; the 3-D position r3x, r3y, r3z; face; neighbour count nbr [n];
+ * phi[2] = (phi_ref, phi_polar) of the step; flags[1] = TISSUE_FLAG_* raised, which a call clears
+ * first). Outputs must not alias inputs. The mesh's device tables must already be on this GPU
+ * (any tissue_create on the mesh puts them there). The work is enqueued on `stream` (a
+ * hipStream_t; NULL: the default stream) and the call returns without waiting for it.
+ * Every buffer beyond the caller's lives in a static device array of
+ * TISSUE_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until the process ends:
+ * calls on one GPU share it, so calls that overlap on one GPU race. Sizes beyond
+ * TISSUE_NOALLOC_MAX_PARTICLES particles or TISSUE_NOALLOC_MAX_UV_VERTICES UV vertices are
+ * rejected with TISSUE_E_INVALID_ARG. */
+#define TISSUE_NOALLOC_MAX_PARTICLES 16384
+#define TISSUE_NOALLOC_MAX_UV_VERTICES 32768
+#define TISSUE_NOALLOC_SCRATCH_BYTES (5u << 20)
+int32_t tissue_step_noalloc(const tissue_mesh* mesh, const tissue_params* p, int32_t n, const float* x_in,
+                            const float* y_in, const float* theta_in, const int32_t* vid_in, const float* noise,
+                            float* x_out, float* y_out, float* theta_out, int32_t* vid_out, float* vx, float* vy,
+                            float* r3x, float* r3y, float* r3z, int32_t* face, int32_t* nbr, double* phi,
+                            uint32_t* flags, void* stream);
 
 /* ---- host helpers (no GPU) ---- */
 /* The area-uniform initial state that tissue_init_random(seed) draws on the GPU, computed on the
