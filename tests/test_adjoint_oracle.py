@@ -141,3 +141,68 @@ def test_isolated_particle_closed_form(O, c1):
     expect = np.mean(2 * 0.5 * p["v0"] * a * np.sin(thetas))
     assert abs(g["d_theta0"][0] - expect) <= 1e-12
     assert abs(g["d_v0"] - np.mean(2 * 0.5 * (p["v0"] - a * np.cos(thetas)))) <= 1e-12
+
+
+def c4_setup(T, O):
+    """The differentiation workload: 1,000 particles on the frequency-32 ellipsoid, sigma 0.05,
+    initial state of seed 2 (tissue_init_random's draw, computed on the host), noise seed 2
+    (std 0.1), loss (phi_T - 0.8)^2 + 0.1 mean |rdot - u|^2 with the target field of seed 4."""
+    m, tb = P.mesh_and_tables(32, 0.05)
+    om = O.Mesh(tb)
+    n = 1000
+    x, y, th = (a.astype(np.float64) for a in m.random_init_host(2, n))
+    _, vid, _ = O.remap(om, x, y, 1)
+    fm = T.target_field(4)
+    comp, amp, kx, ky, ph = (np.array(v) for v in zip(*fm))
+    L = O.Loss(0.8, 1.0, 0.1, component=comp, amp=amp, kx=kx, ky=ky, phase=ph)
+    p = dict(P.PHYS, sigma=0.05)
+    noise = T.gen_noise(2, 0.1, 8, n).astype(np.float64)
+    return om, p, L, (x, y, th, vid), noise
+
+
+def sparse_run(O, om, p, s0, noise, L, dth=None):
+    """FP64 forward (the oracle's O(N) step, bitwise its dense one) and the loss of its trajectory."""
+    x, y, th, vid = s0
+    s = dict(x=x, y=y, theta=th if dth is None else th + dth, vid=vid)
+    keys = ("x", "y", "theta", "vid")
+    hist = {k: [] for k in keys}
+    vx, vy = [], []
+    for t in range(len(noise)):
+        for k in keys:
+            hist[k].append(s[k])
+        st, s, out = O.step_sparse(om, p, s["x"], s["y"], s["theta"], s["vid"], noise[t])
+        assert st == O.OK
+        vx.append(out["vx"]); vy.append(out["vy"])
+    for k in keys:
+        hist[k].append(s[k])
+    states = {k: np.array(v) for k, v in hist.items()}
+    return O.trajectory_loss(states, (np.array(vx), np.array(vy)), L)["loss"], states
+
+
+FD_STEPS = (1e-5, 3e-6, 1e-6, 3e-7, 1e-7, 3e-8, 1e-8, 3e-9, 1e-9, 3e-10, 1e-10, 3e-11, 1e-11)
+
+
+def test_oracle_adjoint_matches_fd_differentiation_workload(T, O):
+    """On the differentiation workload itself, the reverse pass equals central differences of the
+    FP64 forward within 1e-4 for d tau, d mu, d v0 and a random direction of the initial headings,
+    over 4 steps: the longest horizon at which, for all four, a step size from FD_STEPS keeps every
+    vertex id of every step (at 5 steps d v0 agrees only to 1.5e-4, and from 6 steps on no step
+    size keeps the decisions, as the d == 0 rule multiplies sensitivities about 10x per step)."""
+    om, p, L, s0, noise = c4_setup(T, O)
+    H = 4
+    base, states = sparse_run(O, om, p, s0, noise[:H], L)
+    g = O.adjoint_reverse(om, p, states, L)
+    assert abs(g["loss"] - base) <= 1e-12 * abs(base)
+    errs = {}
+    for name, key in (("tau", "d_tau"), ("mu", "d_mu"), ("v0", "d_v0")):
+        def f(h, name=name):
+            return sparse_run(O, om, dict(p, **{name: p[name] + h}), s0, noise[:H], L)
+
+        fd = central_difference(f, states["vid"], FD_STEPS)
+        errs[key] = abs(fd - g[key]) / abs(fd)
+    v = np.random.default_rng(11).standard_normal(len(s0[0]))
+    fd = central_difference(lambda h: sparse_run(O, om, p, s0, noise[:H], L, dth=h * v), states["vid"],
+                            tuple(h * 0.1 for h in FD_STEPS))
+    errs["d_theta0"] = abs(fd - g["d_theta0"] @ v) / abs(fd)
+    print(errs, {k: g[k] for k in ("loss", "d_tau", "d_mu", "d_v0")}, flush=True)
+    assert all(e <= 1e-4 for e in errs.values()), errs

@@ -224,3 +224,41 @@ def test_torch_autograd_wrapper(T):
     with torch.no_grad():
         L2 = TrajectoryLoss.apply(theta0 - lr * theta0.grad, tau, mu, v0, cfg)
     assert L2.item() < L.item()
+
+
+def test_gpu_adjoint_matches_fd_differentiation_workload(T, O):
+    """The GPU adjoint on the differentiation workload (1,000 particles, frequency-32 ellipsoid,
+    init and noise seed 2, target field of seed 4) against central differences of the FP64
+    oracle's forward from the same initial state: within 1e-4 for d tau, d mu, d v0 and a random
+    direction of the initial headings. The horizon is 2 steps, the longest at which the GPU's FP32
+    trajectory makes the FP64 trajectory's decisions (a vertex id differs at step 3), so that both
+    differentiate the same map; beyond it the GPU gradient is checked against the FP64 reverse
+    pass along its own trajectory (above), and that reverse pass against central differences up
+    to 4 steps (test_adjoint_oracle.py)."""
+    from test_adjoint_oracle import FD_STEPS, c4_setup, central_difference, sparse_run
+    om, p, Lo, s0, noise = c4_setup(T, O)
+    H, n = 2, len(s0[0])
+    m, _ = P.mesh_and_tables(32, 0.05)
+    Lc = T.loss(0.8, 1.0, 0.1, T.target_field(4))
+    sim = make(T, m, p, n, 2)
+    st0 = sim.get_state()
+    assert np.array_equal(st0.x.astype(np.float64), s0[0]) and np.array_equal(st0.vid, s0[3])
+    sim.stage_noise(noise[:H].astype(np.float32))
+    code, g, dth, _, _ = sim.adjoint(H, Lc, 0, 20)
+    assert code == T.OK
+    gpu_states = record(T, make(T, m, p, n, 2), noise[:H].astype(np.float32))
+    base, states = sparse_run(O, om, p, s0, noise[:H], Lo)
+    assert np.array_equal(gpu_states["vid"], states["vid"])  # the same decisions at every step
+    errs = {}
+    for name, key in (("tau", "d_tau"), ("mu", "d_mu"), ("v0", "d_v0")):
+        def f(h, name=name):
+            return sparse_run(O, om, dict(p, **{name: p[name] + h}), s0, noise[:H], Lo)
+
+        fd = central_difference(f, states["vid"], FD_STEPS)
+        errs[key] = abs(fd - g[key]) / abs(fd)
+    v = np.random.default_rng(11).standard_normal(n)
+    fd = central_difference(lambda h: sparse_run(O, om, p, s0, noise[:H], Lo, dth=h * v), states["vid"],
+                            tuple(h * 0.1 for h in FD_STEPS))
+    errs["d_theta0"] = abs(fd - dth @ v) / abs(fd)
+    print(errs, {k: g[k] for k in ("loss", "d_tau", "d_mu", "d_v0")}, flush=True)
+    assert all(e <= TOL for e in errs.values()), errs
