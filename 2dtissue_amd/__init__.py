@@ -223,8 +223,9 @@ class Mesh:
         return cls(h)
 
     def build_tables(self, sigma_max: float, grid: int = 0, where: int | None = None) -> "Mesh":
-        """Face buckets and geodesic table (tissue_mesh_build_tables); where = BUILD_HOST or
-        BUILD_GPU places the table's marches (tissue_mesh_build_tables_on), None: automatic."""
+        """Face buckets and the exact geodesic table (tissue_mesh_build_tables); where = BUILD_HOST
+        or BUILD_AUTO through tissue_mesh_build_tables_on (BUILD_GPU is rejected: the table is built
+        on the host's threads), None: automatic."""
         if where is None:
             _check(lib().tissue_mesh_build_tables(self.h, sigma_max, grid))
         else:

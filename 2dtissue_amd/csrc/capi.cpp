@@ -12,7 +12,6 @@
 #include <vector>
 
 #include "capi_common.h"
-#include "mesh_gpu.h"
 #include "rng.h"
 
 using namespace tcapi;
@@ -99,19 +98,13 @@ int32_t tissue_mesh_build_tables_on(tissue_mesh* m, double sigma_max, int32_t gr
   return guarded([&] {
     if (!m || !(sigma_max > 0) || !std::isfinite(sigma_max)) throw ApiError(TISSUE_E_INVALID_ARG, "bad table arguments");
     if (where < TISSUE_BUILD_AUTO || where > TISSUE_BUILD_GPU) throw ApiError(TISSUE_E_INVALID_ARG, "bad build place");
+    if (where == TISSUE_BUILD_GPU)
+      throw ApiError(TISSUE_E_INVALID_ARG,
+                     "the geodesic table is built on the host's threads (TISSUE_BUILD_HOST or TISSUE_BUILD_AUTO)");
     std::lock_guard<std::mutex> lk(m->mu);
     if (m->dev_ready) throw ApiError(TISSUE_E_INVALID_ARG, "mesh tables are in use on a device");
-    int ndev = 0;
-    const bool have_gpu = hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0;
-    if (where == TISSUE_BUILD_GPU && !have_gpu) throw ApiError(TISSUE_E_INVALID_ARG, "no GPU for the table build");
-    const bool gpu = where == TISSUE_BUILD_GPU || (where == TISSUE_BUILD_AUTO && have_gpu);
     tissue::build_face_buckets(m->m, grid);
-    try {
-      tissue::build_distance_table(m->m, 2.4 * sigma_max, gpu ? tissue::DeviceMarch(tissue::march_on_gpu) : nullptr);
-    } catch (const std::runtime_error& e) {
-      if (gpu) throw ApiError(TISSUE_E_HIP, e.what());
-      throw;
-    }
+    tissue::build_distance_table(m->m, 2.4 * sigma_max);
     m->sigma_max = sigma_max;
     m->tables = true;
     return TISSUE_OK;

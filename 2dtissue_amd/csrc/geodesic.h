@@ -1,72 +1,90 @@
-// Front updates of the geodesic distance table (local fast marching on the triangle mesh),
-// shared by the host march (mesh.cpp, built by the host compiler) and the GPU march
-// (mesh_gpu.hip): one source text, compiled without contraction on both sides, so both round
-// every operation alike (IEEE add, multiply, divide and square root).
+// Exact geodesic distances on a triangle mesh, truncated at a radius: the distance table's rows.
+//
+// The distance between two vertices is the length of the shortest path on the piecewise-flat
+// surface (the polyhedral geodesic distance). It is computed per source vertex by window
+// propagation (Chen & Han 1990; windows as in Surazhsky et al., "Fast exact and approximate
+// geodesics on meshes", SIGGRAPH 2005; processed in order of their least distance, as in Xin &
+// Wang, "Improving Chen and Han's algorithm on the discrete geodesic problem", 2009):
+//   - a window is an interval [b0, b1] of a mesh edge together with the unfolded image of the
+//     source (or of a pseudo-source) in the plane of the face it enters: every point t of the
+//     interval is reached by the straight segment from the image, at distance sigma + |image - t|;
+//   - propagating a window across its face unfolds the face's third vertex c into the edge's
+//     plane; the rays from the image through the interval hit c (which then gets its distance)
+//     and split into the parts that leave through the two other edges (the child windows);
+//   - shortest paths bend only at saddle vertices (angle sum above 2 pi) and at boundary
+//     vertices (the mesh is cut open along the seam); those become pseudo-sources that start
+//     windows of their own, with sigma = their distance;
+//   - a window is dropped when one of its edge's endpoints reaches every point of it by a
+//     shorter path (the endpoint's distance plus the straight path along the edge): then no
+//     shortest path runs through the window. The distance of the window over its interval minus
+//     that alternative is monotone along the edge, so the test at one end of the interval covers
+//     all of it;
+//   - events (windows and pseudo-sources) are processed in order of their least distance, so a
+//     pseudo-source emits once, with its final distance, and everything beyond the radius stops.
+// Only vertex-to-vertex distances are kept. Everything is FP64.
 #pragma once
+#include <algorithm>
 #include <cmath>
-
-#if defined(__HIPCC__)
-#define TISSUE_GEO_HD __host__ __device__
-#else
-#define TISSUE_GEO_HD
-#endif
+#include <cstdint>
+#include <limits>
+#include <queue>
+#include <utility>
+#include <vector>
 
 namespace tissue {
 namespace geo {
 
-TISSUE_GEO_HD inline double inf() { return __builtin_inf(); }
-TISSUE_GEO_HD inline void sub3(const double* a, const double* b, double* o) {
-  o[0] = a[0] - b[0];
-  o[1] = a[1] - b[1];
-  o[2] = a[2] - b[2];
-}
-TISSUE_GEO_HD inline double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
-TISSUE_GEO_HD inline double norm3(const double* a) { return sqrt(dot3(a, a)); }
-
-// first-order fast-marching update of vertex C from triangle (A, B, C) with known arrival
-// times tA, tB (Kimmel & Sethian 1998, written for 3D edge vectors): the planar front
-// T(P) = tC + g.(P - C), |g| = 1, through A and B; accepted only when the characteristic
-// reaching C enters the triangle through edge AB.
-TISSUE_GEO_HD inline double tri_update(const double* A, double tA, const double* B, double tB, const double* C) {
-  double e1[3], e2[3];
-  sub3(A, C, e1);
-  sub3(B, C, e2);
-  const double m00 = dot3(e1, e1), m01 = dot3(e1, e2), m11 = dot3(e2, e2);
-  const double det = m00 * m11 - m01 * m01;
-  if (!(det > 1e-300)) return inf();
-  const double i00 = m11 / det, i01 = -m01 / det, i11 = m00 / det;
-  const double a = i00 + 2 * i01 + i11;
-  const double b = (i00 + i01) * tA + (i01 + i11) * tB;
-  const double c = i00 * tA * tA + 2 * i01 * tA * tB + i11 * tB * tB - 1.0;
-  const double disc = b * b - a * c;
-  if (!(a > 0) || disc < 0) return inf();
-  const double T = (b + sqrt(disc)) / a;
-  const double r0 = tA - T, r1 = tB - T;
-  const double c0 = i00 * r0 + i01 * r1, c1 = i01 * r0 + i11 * r1;
-  const double tol = 1e-12;
-  const double tmax = tA < tB ? tB : tA;
-  if (c0 > tol || c1 > tol || T < tmax) return inf();
-  return T;
+inline double dist3(const double* a, const double* b) {
+  const double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+  return std::sqrt(dx * dx + dy * dy + dz * dz);
 }
 
-// length of the straight path from V to X across edge (A, B) once the two faces (V,A,B) and
-// (A,B,X) are unfolded into one plane; infinity when that segment misses the edge
-TISSUE_GEO_HD inline double unfold_across(const double* V, const double* A, const double* B, const double* X) {
-  double ab[3], av[3], ax[3];
-  sub3(B, A, ab);
-  sub3(V, A, av);
-  sub3(X, A, ax);
-  const double L = norm3(ab);
-  if (!(L > 0)) return inf();
-  const double vu = dot3(av, ab) / L, xu = dot3(ax, ab) / L;
-  const double hv = dot3(av, av) - vu * vu, hx = dot3(ax, ax) - xu * xu;
-  const double vh = sqrt(0.0 < hv ? hv : 0.0);
-  const double xh = sqrt(0.0 < hx ? hx : 0.0);
-  if (!(vh + xh > 0)) return inf();
-  const double t = vu + (xu - vu) * vh / (vh + xh);  // crossing point along AB
-  if (t < 0 || t > L) return inf();
-  return sqrt((xu - vu) * (xu - vu) + (xh + vh) * (xh + vh));
-}
+// the mesh as the propagation reads it
+struct WindowMesh {
+  int nv = 0, nf = 0;
+  const double* xyz = nullptr;     // nv * 3
+  const int32_t* faces = nullptr;  // nf * 3
+  std::vector<int32_t> across;     // 3 * nf: face across the edge opposite corner q of face f, -1 on the boundary
+  std::vector<int32_t> vf_ptr, vf_idx;  // faces around each vertex (CSR)
+  std::vector<char> pseudo;        // saddle or boundary vertex
+  bool manifold = true;            // no edge with more than two faces
+};
+
+// adjacency, faces around each vertex, and the pseudo-source vertices
+WindowMesh window_mesh(int nv, int nf, const double* xyz, const int32_t* faces);
+
+// one source's propagation (one per thread: it keeps its scratch between sources)
+class WindowPropagation {
+ public:
+  WindowPropagation(const WindowMesh& m, double lim);
+  // (vertex, distance) of every vertex within lim of source s, ascending in (distance, vertex)
+  void run(int s, std::vector<std::pair<int, double>>& row);
+
+ private:
+  struct Window {
+    int face, corner;  // the face the window leaves and its corner opposite the window's edge
+    double b0, b1;     // the interval, along the edge from its first vertex
+    double sx, sy;     // the source image in the edge's frame (sy < 0: behind the edge)
+    double sigma;      // distance of the image from the source
+  };
+  struct Event {
+    double key;  // least distance of any point the event reaches
+    int id;      // window index, or -1 - v for the pseudo-source vertex v
+    bool operator>(const Event& o) const { return key > o.key; }
+  };
+  void relax(int v, double t);
+  void emit(int v, double sigma);
+  void push_window(int f, int corner, double b0, double b1, double sx, double sy, double sigma);
+  void propagate(const Window& w);
+  int corner_vertex(int f, int q) const { return m_.faces[3 * f + q % 3]; }
+
+  const WindowMesh& m_;
+  double lim_;
+  std::vector<double> d_;
+  std::vector<int> touched_;
+  std::vector<Window> windows_;
+  std::priority_queue<Event, std::vector<Event>, std::greater<Event>> queue_;
+};
 
 }  // namespace geo
 }  // namespace tissue

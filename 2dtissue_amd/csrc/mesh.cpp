@@ -662,120 +662,25 @@ int build_threads() {
   return (int)std::clamp<unsigned>(h == 0 ? 1u : h, 1u, 16u);
 }
 
-void build_distance_table(UVMesh& m, double cutoff, const DeviceMarch& device_march) {
+void build_distance_table(UVMesh& m, double cutoff) {
   if (!(cutoff > 0)) throw std::runtime_error("distance table: cutoff must be positive");
   const int nv = m.nv;
   const double lim = cutoff * (1.0 + 1e-6);
-  std::vector<std::vector<int>> vf(nv);
-  for (int f = 0; f < m.nf; ++f)
-    for (int q = 0; q < 3; ++q) vf[m.faces[3 * f + q]].push_back(f);
-
-  // the faces on each edge: (edge key, face) pairs sorted, grouped by edge (faces ascending);
-  // edge_of[3 f + q] is the group of the edge opposite corner q of face f
-  auto edge_key = [](int a, int b) { return a < b ? ((int64_t)a << 32) | b : ((int64_t)b << 32) | a; };
-  std::vector<std::pair<int64_t, int>> ek(3 * (size_t)m.nf);
-  for (int f = 0; f < m.nf; ++f)
-    for (int q = 0; q < 3; ++q)
-      ek[3 * (size_t)f + q] = {edge_key(m.faces[3 * f + (q + 1) % 3], m.faces[3 * f + (q + 2) % 3]), 3 * f + q};
-  std::sort(ek.begin(), ek.end());
-  std::vector<int32_t> edge_ptr{0}, edge_faces, edge_of(3 * (size_t)m.nf);
-  edge_faces.reserve(ek.size());
-  bool manifold = true;
-  for (size_t k = 0; k < ek.size();) {
-    size_t e = k;
-    for (; e < ek.size() && ek[e].first == ek[k].first; ++e) {
-      edge_faces.push_back(ek[e].second / 3);
-      edge_of[ek[e].second] = (int32_t)edge_ptr.size() - 1;
-    }
-    if (e - k > 2) manifold = false;
-    edge_ptr.push_back((int32_t)edge_faces.size());
-    k = e;
-  }
-  MarchRows rows(nv);
-  std::vector<char> done(nv, 0);
-  if (device_march && manifold) {
-    // the device takes the sources first (manifold meshes: at most one face across an edge);
-    // the host marches whatever it leaves
-    MarchMesh mm;
-    mm.vf_ptr.assign(nv + 1, 0);
-    for (int v = 0; v < nv; ++v) mm.vf_ptr[v + 1] = mm.vf_ptr[v] + (int32_t)vf[v].size();
-    mm.vf_idx.reserve(mm.vf_ptr[nv]);
-    for (int v = 0; v < nv; ++v) mm.vf_idx.insert(mm.vf_idx.end(), vf[v].begin(), vf[v].end());
-    mm.across.assign(3 * (size_t)m.nf, -1);
-    for (int f = 0; f < m.nf; ++f)
-      for (int q = 0; q < 3; ++q) {
-        const int g = edge_of[3 * (size_t)f + q];
-        for (int k = edge_ptr[g]; k < edge_ptr[g + 1]; ++k)
-          if (edge_faces[k] != f) mm.across[3 * (size_t)f + q] = edge_faces[k];
-      }
-    device_march(m, mm, lim, rows, done);
-  }
-  using QE = std::pair<double, int>;
-  // one local march per source vertex; sources are independent, so threads take them in turn,
-  // each with its own arrival times (the rows come out the same whatever the thread count)
+  const geo::WindowMesh wm = geo::window_mesh(m.nv, m.nf, m.xyz.data(), m.faces.data());
+  if (!wm.manifold) throw std::runtime_error("distance table: an edge has more than two faces");
+  // one propagation per source vertex; sources are independent, so threads take them in turn,
+  // each with its own scratch (the rows come out the same whatever the thread count)
+  std::vector<std::vector<std::pair<int, double>>> rows(nv);
   std::atomic<int> next_source{0};
-  auto march = [&]() {
-    std::vector<double> T(nv, std::numeric_limits<double>::infinity());
-    std::vector<char> alive(nv, 0);
-    std::vector<int> touched;
-    for (int s = next_source++; s < nv; s = next_source++) {
-      if (done[s]) continue;
-      std::priority_queue<QE, std::vector<QE>, std::greater<QE>> pq;
-      auto relax = [&](int tgt, double cand) {
-        if (cand < T[tgt]) {
-          if (T[tgt] == std::numeric_limits<double>::infinity()) touched.push_back(tgt);
-          T[tgt] = cand;
-          pq.push({cand, tgt});
-        }
-      };
-      T[s] = 0;
-      touched.push_back(s);
-      pq.push({0.0, s});
-      while (!pq.empty()) {
-        auto [tv, v] = pq.top();
-        pq.pop();
-        if (alive[v] || tv > T[v]) continue;
-        if (tv > lim) break;
-        alive[v] = 1;
-        rows[s].push_back({v, tv});
-        const double* X = &m.xyz[3 * v];
-        for (int f : vf[v]) {
-          const int32_t* fc = &m.faces[3 * f];
-          for (int q = 0; q < 3; ++q) {
-            const int tgt = fc[q];
-            if (tgt == v || alive[tgt]) continue;
-            const int oth = fc[0] + fc[1] + fc[2] - v - tgt;
-            double d[3];
-            geo::sub3(&m.xyz[3 * tgt], X, d);
-            double cand = tv + geo::norm3(d);
-            if (alive[oth]) cand = std::min(cand, geo::tri_update(X, tv, &m.xyz[3 * oth], T[oth], &m.xyz[3 * tgt]));
-            relax(tgt, cand);
-          }
-          // straight segment from v across the opposite edge (a, b) into the neighbouring face
-          const int a = fc[0] == v ? fc[1] : fc[0];
-          const int b = fc[0] + fc[1] + fc[2] - v - a;
-          const int qv = fc[0] == v ? 0 : (fc[1] == v ? 1 : 2);
-          const int eg = edge_of[3 * (size_t)f + qv];
-          for (int k = edge_ptr[eg]; k < edge_ptr[eg + 1]; ++k) {
-            const int g2 = edge_faces[k];
-            if (g2 == f) continue;
-            const int32_t* gc = &m.faces[3 * g2];
-            const int x = gc[0] + gc[1] + gc[2] - a - b;
-            if (alive[x]) continue;
-            double L = geo::unfold_across(X, &m.xyz[3 * a], &m.xyz[3 * b], &m.xyz[3 * x]);
-            relax(x, tv + L);
-          }
-        }
-      }
-      for (int v : touched) { T[v] = std::numeric_limits<double>::infinity(); alive[v] = 0; }
-      touched.clear();
-    }
+  auto propagate = [&]() {
+    geo::WindowPropagation wp(wm, lim);
+    for (int s = next_source++; s < nv; s = next_source++) wp.run(s, rows[s]);
   };
   {
     std::vector<std::thread> pool;
     const int nt = build_threads();
-    for (int k = 1; k < nt; ++k) pool.emplace_back(march);
-    march();
+    for (int k = 1; k < nt; ++k) pool.emplace_back(propagate);
+    propagate();
     for (auto& t : pool) t.join();
   }
   // symmetrize with the minimum of both directions (transform_into_symmetric_matrix,

@@ -11,7 +11,6 @@
 #pragma once
 #include <cstdint>
 #include <string>
-#include <functional>
 #include <utility>
 #include <vector>
 
@@ -58,21 +57,8 @@ UVMesh cut_and_parameterize(const SurfaceMesh& m, int pa = -1, int pm = -1, int 
 
 // ---- acceleration structures ----
 void build_face_buckets(UVMesh& uv, int grid = 0);        // grid <= 0: automatic
-// rows of the local marches of the distance table: rows[s] = (vertex, arrival time) in the
-// order the march from source s accepts them
-using MarchRows = std::vector<std::vector<std::pair<int, double>>>;
-// adjacency a march reads: the faces around each vertex (CSR, ascending face ids) and, per face f
-// and corner q, the face across the edge opposite corner q (-1 on the cut)
-struct MarchMesh {
-  std::vector<int32_t> vf_ptr, vf_idx, across;
-};
-// marches on a device: fills rows[s] and sets done[s] for the sources it completed (the host
-// marches the others); must give the host march's rows exactly
-using DeviceMarch = std::function<void(const UVMesh&, const MarchMesh&, double lim, MarchRows& rows,
-                                       std::vector<char>& done)>;
-// geodesic distance on the open mesh, truncated at cutoff: fast marching (first-order planar
-// front updates) combined with exact straight paths across one edge into the next face; the
-// marches from the sources run on device_march when given, else on the host's threads
-void build_distance_table(UVMesh& uv, double cutoff, const DeviceMarch& device_march = nullptr);
+// geodesic distance on the open mesh, truncated at cutoff: exact polyhedral distances by window
+// propagation (geodesic.h), one source per task on the host's threads
+void build_distance_table(UVMesh& uv, double cutoff);
 
 }  // namespace tissue

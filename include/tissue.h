@@ -68,15 +68,15 @@ int32_t tissue_mesh_perturbed_sphere(double amp, int32_t freq, tissue_mesh** out
 /* closed genus-0 triangle mesh from an OFF file (loadMeshFaces + create_uv_surface, 2DTissue.cpp:85-91) */
 int32_t tissue_mesh_load_off(const char* path, tissue_mesh** out);
 /* face buckets (grid <= 0: automatic) and the geodesic table on the open mesh, truncated at
- * 2.4 * sigma_max (CachedGeodesicDistanceHelper(<stem>_open.off), 2DTissue.cpp:102-105); the
- * table's local marches run on the current GPU when one is visible, else on the host's threads
- * (the same table either way) */
+ * 2.4 * sigma_max (CachedGeodesicDistanceHelper(<stem>_open.off), 2DTissue.cpp:102-105): the exact
+ * polyhedral geodesic distance between vertices (shortest path on the piecewise-flat surface, by
+ * window propagation on the host's threads), rounded to FP32 */
 int32_t tissue_mesh_build_tables(tissue_mesh* m, double sigma_max, int32_t grid);
-/* where the marches of tissue_mesh_build_tables_on run */
+/* where tissue_mesh_build_tables_on builds the table */
 enum { TISSUE_BUILD_AUTO = -1, TISSUE_BUILD_HOST = 0, TISSUE_BUILD_GPU = 1 };
-/* tissue_mesh_build_tables with the place of the marches chosen (TISSUE_BUILD_GPU: the current
- * GPU, an error when none is visible; marches that outgrow the kernel's tables still run on the
- * host). Replaces the same reference call as tissue_mesh_build_tables. */
+/* tissue_mesh_build_tables with the place chosen: TISSUE_BUILD_HOST and TISSUE_BUILD_AUTO build on
+ * the host's threads; TISSUE_BUILD_GPU is rejected (TISSUE_E_INVALID_ARG): the table has no GPU
+ * build. Replaces the same reference call as tissue_mesh_build_tables. */
 int32_t tissue_mesh_build_tables_on(tissue_mesh* m, double sigma_max, int32_t grid, int32_t where);
 void tissue_mesh_destroy(tissue_mesh* m);
 int32_t tissue_mesh_info(const tissue_mesh* m, int32_t* nv, int32_t* nf, int32_t* nv3d, int64_t* nnz, int32_t* grid,

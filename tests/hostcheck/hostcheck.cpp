@@ -1,8 +1,11 @@
 // Host build of the device geometry (2dtissue_amd/csrc/geom.h) for exactness tests against the
-// FP64 oracle. Test infrastructure: compiled by tests/hostcheck/Makefile, loaded by
-// tests/test_device_math.py; nothing in the library uses it.
+// FP64 oracle, and of the geodesic table's window propagation (geodesic.cpp) for closed-form
+// tests on meshes the library does not build (open sheets). Test infrastructure: compiled by
+// tests/hostcheck/Makefile, loaded by tests/test_device_math.py and tests/test_geodesic.py;
+// nothing in the library uses it.
 #include <vector>
 
+#include "../../2dtissue_amd/csrc/geodesic.h"
 #include "../../2dtissue_amd/csrc/geom.h"
 
 extern "C" {
@@ -91,6 +94,22 @@ void hc_locate_many(int n, const double* pts, int nv, int nf, const float* uv, c
     r3d[3 * i + 2] = L.Z;
     fp64[i] = L.fp64;
   }
+}
+
+
+// row of source s of the exact truncated geodesic distances on a triangle mesh (any manifold
+// mesh, open or closed): writes up to cap (vertex, distance) pairs, returns the row length
+int hc_geodesic_row(int nv, int nf, const double* xyz, const int* faces, double lim, int s, int cap, int* out_v,
+                    double* out_d) {
+  const tissue::geo::WindowMesh m = tissue::geo::window_mesh(nv, nf, xyz, faces);
+  tissue::geo::WindowPropagation wp(m, lim);
+  std::vector<std::pair<int, double>> row;
+  wp.run(s, row);
+  for (int k = 0; k < (int)row.size() && k < cap; ++k) {
+    out_v[k] = row[k].first;
+    out_d[k] = row[k].second;
+  }
+  return (int)row.size();
 }
 
 }  // extern "C"

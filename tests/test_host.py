@@ -142,26 +142,6 @@ def test_mesh_build_independent_of_thread_count(tmp_path):
         assert out[1][k].tobytes() == out[5][k].tobytes(), k
 
 
-def test_geodesic_table_vs_chord_and_sphere(T):
-    """On the unit sphere the table must lie between the chord (a lower bound) and the
-    great-circle distance plus the mesh's discretisation error."""
-    m = T.Mesh.ellipsoid(1.0, 1.0, 1.0, 16).build_tables(0.15)
-    t = m.export()
-    xyz, rp, col, dist = t["xyz"], t["row_ptr"], t["col"], t["dist"]
-    ratios = []
-    for v in range(0, t["nv"], 7):
-        for e in range(rp[v], rp[v + 1]):
-            w = col[e]
-            ch = np.linalg.norm(xyz[v] - xyz[w])
-            if w == v or ch == 0 or dist[e] > 0.3:
-                continue
-            gc = 2 * math.asin(min(1.0, ch / 2))
-            assert dist[e] >= ch * (1 - 1e-6)
-            ratios.append(dist[e] / gc)
-    r = np.array(ratios)
-    assert np.median(r) < 1.01 and np.percentile(r, 99) < 1.08
-
-
 def test_off_roundtrip(T, ell):
     m, t = ell
     with tempfile.TemporaryDirectory() as d:
@@ -205,10 +185,9 @@ def test_mesh_errors(T):
         m.build_tables(-0.1, where=T.BUILD_HOST)
     m.build_tables(0.1, where=T.BUILD_HOST)
     assert m.info()["nnz"] > 0
-    try:  # the GPU march is an error exactly when no GPU is visible
+    with pytest.raises(T.TissueError) as e:  # the table has no GPU build
         T.Mesh.ellipsoid(2.0, 1.0, 1.0, 6).build_tables(0.1, where=T.BUILD_GPU)
-    except T.TissueError as e:
-        assert "no GPU" in str(e)
+    assert "host" in str(e.value)
 
 
 def restated_init(T, tb, seed, n):
