@@ -48,8 +48,12 @@ struct BucketsIn {
 };
 // The same buckets while the step fills them: the first kSlots arrivals of a vertex take its
 // slots, later ones go to the (vertex, id) overflow list; the second arrival at a vertex also
-// lists the vertex in multi (the buckets the finalize pass must sort).
-struct BucketsOut { int* count; int* slot; int* ovf_n; int2* ovf; int* multi_n; int* multi; };
+// lists the vertex in multi (the buckets the finalize pass must sort); the first arrival sets the
+// vertex's bit in the occupancy mask occ (bit v % 32 of word v / 32).
+struct BucketsOut { int* count; int* slot; int* ovf_n; int2* ovf; int* multi_n; int* multi; unsigned* occ; };
+
+// words of a vertex-occupancy mask over nv vertices
+__host__ __device__ constexpr int occ_words(int nv) { return (nv + 31) >> 5; }
 
 // ------------------------------------------------------------------ neighbour traversal
 
@@ -136,11 +140,18 @@ constexpr int kHeadScanLanes = 8;
 constexpr int kHeadChunk = 2;
 static_assert(32 % kScanChunk == 0, "scan chunks must tile the 32-entry window");
 
-// Global-memory neighbour source: the previous step's SoA state and vertex buckets.
+// Global-memory neighbour source: the previous step's SoA state and vertex buckets. With OCC the
+// buckets' occupancy mask is staged in LDS by the workgroup: the table walk tests occupancy there
+// and touches a bucket in global memory only when its vertex holds particles; without it the walk
+// reads each entry's bucket count. Both visit the same buckets in the same order.
+template <bool OCC>
 struct GlobalSource {
+  static constexpr bool kOcc = OCC;
   BucketsIn b;
   const float4* xycs;
   const float* th;
+  const unsigned* occ;  // LDS copy of the replica's occupancy mask of b (OCC only)
+  __device__ __forceinline__ bool occupied(int w) const { return (occ[w >> 5] >> (w & 31)) & 1u; }
   struct Head { int c = 0, s0 = 0, s1 = 0; };
   // bucket head: count and the first two slots, loaded together (the slots are in bounds
   // whatever the count)
@@ -183,7 +194,9 @@ struct LdsSource {
     h.c = (int)begin[w + 1] - h.b;
     return h;
   }
+  static constexpr bool kOcc = false;  // counts are LDS reads already
   __device__ __forceinline__ int count(int w) const { return (int)begin[w + 1] - (int)begin[w]; }
+  __device__ __forceinline__ bool occupied(int w) const { return begin[w + 1] != begin[w]; }
   template <class F>
   __device__ __forceinline__ void each(int, const Head& h, F&& f) const {
     const int b = h.b, c = h.c;
@@ -257,11 +270,15 @@ __device__ __forceinline__ StepOut particle_step(const MeshDev& mesh, const Step
         const int e = base + k * LANES;
         ent[k] = e < re ? mesh.nbr[e] : make_int2(0, 0x7f800000);  // +inf ends the walk
       }
+      bool oc[kHeadChunk];
+#pragma unroll
+      for (int k = 0; k < kHeadChunk; ++k)
+        oc[k] = __int_as_float(ent[k].y) <= prm.t_count && (!Src::kOcc || src.occupied(ent[k].x));
 #pragma unroll
       for (int k = 0; k < kHeadChunk; ++k) {
         const float d = __int_as_float(ent[k].y);
-        if (d <= prm.t_near) hd[k] = src.head(ent[k].x);
-        else if (d <= prm.t_count) hd[k].c = src.count(ent[k].x);
+        if (oc[k] && d <= prm.t_near) hd[k] = src.head(ent[k].x);
+        else if (oc[k]) hd[k].c = src.count(ent[k].x);
       }
       bool stop = false;
 #pragma unroll
@@ -278,9 +295,10 @@ __device__ __forceinline__ StepOut particle_step(const MeshDev& mesh, const Step
     }
   } else {
     // Windows of 32 entries per lane:
-    //  (1) scan: the window's table entries and their bucket counts are loaded kScanChunk at a
-    //      time (one load latency per chunk), counted-only entries (2 sigma <= d <= 2.4 sigma) add
-    //      their count to nb, and near entries whose bucket holds particles are marked;
+    //  (1) scan: the window's table entries are loaded kScanChunk at a time (one load latency per
+    //      chunk) and their occupancy found (from the LDS mask, or from the bucket counts);
+    //      occupied counted-only entries (2 sigma <= d <= 2.4 sigma) add their bucket count to nb,
+    //      and occupied near entries are marked;
     //  (2) visit: only the marked entries, in order, set up their pair coefficient and add their
     //      particles. Most table entries have empty buckets, so the lanes of a wave loop over a
     //      few occupied entries instead of paying the pair set-up for every entry; the order, and
@@ -296,8 +314,20 @@ __device__ __forceinline__ StepOut particle_step(const MeshDev& mesh, const Step
           const int e = wbase + (m0 + k) * LANES;
           ent[k] = e < re ? mesh.nbr[e] : make_int2(0, 0x7f800000);  // +inf ends the walk
         }
+        bool oc[kScanChunk];
+        if constexpr (Src::kOcc) {
+          // occupancy from LDS; bucket counts only for the occupied entries counted without a
+          // pair (2 sigma <= d <= 2.4 sigma)
 #pragma unroll
-        for (int k = 0; k < kScanChunk; ++k) cnt[k] = __int_as_float(ent[k].y) <= prm.t_count ? src.count(ent[k].x) : 0;
+          for (int k = 0; k < kScanChunk; ++k) oc[k] = __int_as_float(ent[k].y) <= prm.t_count && src.occupied(ent[k].x);
+#pragma unroll
+          for (int k = 0; k < kScanChunk; ++k) cnt[k] = oc[k] && __int_as_float(ent[k].y) > prm.t_near ? src.count(ent[k].x) : 0;
+        } else {
+#pragma unroll
+          for (int k = 0; k < kScanChunk; ++k) cnt[k] = __int_as_float(ent[k].y) <= prm.t_count ? src.count(ent[k].x) : 0;
+#pragma unroll
+          for (int k = 0; k < kScanChunk; ++k) oc[k] = cnt[k] > 0;
+        }
 #pragma unroll
         for (int k = 0; k < kScanChunk; ++k) {
           const float d = __int_as_float(ent[k].y);
@@ -306,7 +336,7 @@ __device__ __forceinline__ StepOut particle_step(const MeshDev& mesh, const Step
             break;
           }
           if (d > prm.t_near) nb += cnt[k];  // counted only
-          else if (cnt[k] > 0) mark |= 1u << (m0 + k);
+          else if (oc[k]) mark |= 1u << (m0 + k);
         }
       }
       while (mark) {
@@ -360,7 +390,7 @@ __device__ __forceinline__ StepOut particle_step(const MeshDev& mesh, const Step
       }
 #pragma unroll
       for (int k = 0; k < kRowChunk; ++k)
-        if (__int_as_float(ent[k].y) <= prm.t_near) head[k] = src.head(ent[k].x);
+        if (__int_as_float(ent[k].y) <= prm.t_near && (!Src::kOcc || src.occupied(ent[k].x))) head[k] = src.head(ent[k].x);
       bool stop = false;
 #pragma unroll
       for (int k = 0; k < kRowChunk; ++k) {

@@ -230,6 +230,7 @@ void Engine::get_outputs(float* ovx, float* ovy, float* r3d, int32_t* oface, int
 void Engine::alloc_bucket_set(BucketSet& b) {
   const size_t cells = (size_t)R * md.nv;
   b.sorted = raw_buckets ? 0 : 1;
+  b.occ = dalloc<unsigned>((size_t)R * tdev::occ_words(md.nv));
   b.count = dalloc<int>(cells);
   b.slot = dalloc<int>(cells * tdev::kSlots);
   b.ovf_n = dalloc<int>(R);
@@ -241,19 +242,20 @@ void Engine::alloc_bucket_set(BucketSet& b) {
 }
 
 void Engine::free_bucket_set(BucketSet& b) {
-  dfree(b.count); dfree(b.slot); dfree(b.ovf_n); dfree(b.ovf);
+  dfree(b.occ); dfree(b.count); dfree(b.slot); dfree(b.ovf_n); dfree(b.ovf);
   dfree(b.multi_n); dfree(b.multi); dfree(b.big_n); dfree(b.big);
 }
 
 size_t Engine::bucket_set_bytes() const {
   return (size_t)R * md.nv * (1 + tdev::kSlots) * sizeof(int) + (size_t)ntot * (sizeof(int2) + 2 * sizeof(int)) +
-         3 * (size_t)R * sizeof(int);
+         3 * (size_t)R * sizeof(int) + (size_t)R * tdev::occ_words(md.nv) * sizeof(unsigned);
 }
 
 // outside any capture: clear set b (count and list lengths), on stream st (null: the engine's)
 void Engine::clear_bucket_set(BucketSet& b, hipStream_t st) {
   if (!st) st = stream;
   HIP_OK(hipMemsetAsync(b.count, 0, (size_t)R * md.nv * sizeof(int), st));
+  HIP_OK(hipMemsetAsync(b.occ, 0, (size_t)R * tdev::occ_words(md.nv) * sizeof(unsigned), st));
   HIP_OK(hipMemsetAsync(b.ovf_n, 0, R * sizeof(int), st));
   HIP_OK(hipMemsetAsync(b.multi_n, 0, R * sizeof(int), st));
   HIP_OK(hipMemsetAsync(b.big_n, 0, R * sizeof(int), st));
@@ -286,6 +288,7 @@ void Engine::rebuild_buckets() {
   const size_t cells = (size_t)R * md.nv;
   for (int k = 0; k < 3; ++k) {
     HIP_OK(hipMemsetAsync(bs[k].count, 0, cells * sizeof(int), stream));
+    HIP_OK(hipMemsetAsync(bs[k].occ, 0, (size_t)R * tdev::occ_words(md.nv) * sizeof(unsigned), stream));
     HIP_OK(hipMemsetAsync(bs[k].ovf_n, 0, R * sizeof(int), stream));
     HIP_OK(hipMemsetAsync(bs[k].multi_n, 0, R * sizeof(int), stream));
     HIP_OK(hipMemsetAsync(bs[k].big_n, 0, R * sizeof(int), stream));
@@ -484,7 +487,7 @@ void Engine::enqueue(const Shape& s, int n_steps, const float* nz, size_t stride
         io.noise = nz ? nz + (size_t)t * stride : nullptr;
         io.noise_row0 = d_row0;
         io.noise_stride = stride;
-        io.partial = partial + (size_t)q * n_logical;
+        io.partial = partial + (size_t)q * n_logical * partials_per_tile(s.block);
         io.halt_in = d_rows + (size_t)(t % 3) * R;
         io.halt_out = d_rows + (size_t)((t + 1) % 3) * R;
         io.halt_clr = d_rows + (size_t)((t + 2) % 3) * R;
@@ -501,7 +504,8 @@ void Engine::enqueue(const Shape& s, int n_steps, const float* nz, size_t stride
         sp = 1 - sp;
         bp = (bp + 1) % 3;
       }
-      HIP_OK(launch_phi_tiles(R, k, d_tile_begin, n_logical, partial, d_reps, phi, n_steps, t0, stream));
+      HIP_OK(launch_phi_tiles(R, k, d_tile_begin, n_logical, partials_per_tile(s.block), partial, d_reps, phi,
+                              n_steps, t0, stream));
     }
   }
   HIP_OK(launch_flags_end(R, d_rows + (size_t)(n_steps % 3) * R, d_halt, stream));
@@ -545,7 +549,7 @@ int32_t Engine::run_once(const Shape& s, int n_steps, const float* nz, size_t st
   if (s.mode == TISSUE_MODE_STEP_KERNEL) {
     ensure_buckets();
     ensure_tiles(s);
-    const size_t need = (size_t)s.spl * n_logical;
+    const size_t need = (size_t)s.spl * n_logical * partials_per_tile(s.block);
     if (partial_cap < need) {
       HIP_OK(hipStreamSynchronize(stream));
       drop_graphs();

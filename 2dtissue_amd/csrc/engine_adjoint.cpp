@@ -140,10 +140,11 @@ void Engine::adjoint_buffers(int K, int C) {
   }
   for (int k = 0; k < ntape; ++k) {
     Tape& t = a.tp[k];
-    if (t.partial_n < n_logical) {
+    const int np = n_logical * partials_per_tile(kStepMaxBlock);  // written, not read: sized for any shape
+    if (t.partial_n < np) {
       dfree(t.partial);
-      t.partial = dalloc<OrderPartial>(n_logical);
-      t.partial_n = n_logical;
+      t.partial = dalloc<OrderPartial>(np);
+      t.partial_n = np;
     }
   }
 }

@@ -23,6 +23,7 @@ namespace {
 
 __device__ __forceinline__ void bucket_insert(const BucketsOut& b, int v, int i) {
   const int old = atomicAdd(&b.count[v], 1);
+  if (old == 0) atomicOr(&b.occ[v >> 5], 1u << (v & 31));
   if (old < kSlots) {
     b.slot[(size_t)v * kSlots + old] = i;
   } else {
@@ -34,7 +35,7 @@ __device__ __forceinline__ void bucket_insert(const BucketsOut& b, int v, int i)
 
 __device__ __forceinline__ BucketsOut replica_out(const BucketSet& s, int r, int nv, int off) {
   return {s.count + (size_t)r * nv, s.slot + (size_t)r * nv * kSlots, s.ovf_n + r, s.ovf + off, s.multi_n + r,
-          s.multi + off};
+          s.multi + off, s.occ + (size_t)r * occ_words(nv)};
 }
 
 // orders the LDS / global accesses of one wave (its lanes exchange data through memory)
@@ -43,39 +44,39 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-// block-wide deterministic sum of (sgn, cs, sn): butterfly within each wave64, then wave
-// partials added in wave order by thread 0
-__device__ void block_order_partial(int sgn, double cs, double sn, OrderPartial* out) {
-  __shared__ double s_cs[16], s_sn[16];
-  __shared__ int s_sg[16];
+// the wave's sums of (sgn, cs, sn) in fixed point, written by its lane 0 to out: each wave keeps
+// its own partial (no workgroup barrier, so a finished wave leaves at once); integer sums make the
+// result independent of their order
+__device__ __forceinline__ void wave_order_partial(int sgn, long long cs, long long sn, OrderPartial* out) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     sgn += __shfl_xor(sgn, off, 64);
     cs += __shfl_xor(cs, off, 64);
     sn += __shfl_xor(sn, off, 64);
   }
-  const int wave = threadIdx.x / 64, nw = (blockDim.x + 63) / 64;
-  if ((threadIdx.x & 63) == 0) { s_cs[wave] = cs; s_sn[wave] = sn; s_sg[wave] = sgn; }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    OrderPartial p{0.0, 0.0, 0, 0};
-    for (int w = 0; w < nw; ++w) { p.cs += s_cs[w]; p.sn += s_sn[w]; p.sgn += s_sg[w]; }
-    *out = p;
-  }
+  if ((threadIdx.x & 63) == 0) *out = OrderPartial{cs, sn, sgn, 0};
 }
 
-template <int LANES, bool TAPE>
+template <int LANES, bool TAPE, bool OCC>
 __global__ void __launch_bounds__(kStepMaxBlock) k_step_tiled(MeshDev mesh, const ReplicaDesc* reps,
                                                               const TileDesc* tiles, BucketSet bin, BucketSet bout,
                                                               BucketSet bclr, TiledIO io) {
+  extern __shared__ unsigned s_occ[];  // OCC: occ_words(nv), the input buckets' occupancy mask
   const TileDesc tile = tiles[blockIdx.x];
   if (tile.r < 0) return;  // idle padding workgroup (whole block)
   const int r = tile.r;
   const ReplicaDesc& rep = reps[r];  // uniform: read through the scalar cache where used
-  const int nv = mesh.nv, tid = threadIdx.x;
+  const int nv = mesh.nv, tid = threadIdx.x, nw = occ_words(nv);
+  if constexpr (OCC) {  // stage this replica's occupancy mask of the input buckets in LDS
+    const unsigned* occ_in = bin.occ + (size_t)r * nw;
+    for (int k = tid; k < nw; k += (int)blockDim.x) s_occ[k] = occ_in[k];
+  }
   // clear this replica's bucket set consumed two steps ago (it is next step's output)
   int* clr = bclr.count + (size_t)r * nv;
   for (int v = tile.q * (int)blockDim.x + tid; v < nv; v += tile.nq * (int)blockDim.x) clr[v] = 0;
+  unsigned* clr_occ = bclr.occ + (size_t)r * nw;
+  for (int k = tile.q * (int)blockDim.x + tid; k < nw; k += tile.nq * (int)blockDim.x) clr_occ[k] = 0u;
+  __syncthreads();
   // a replica flagged before this step passes its state through unchanged (the reference stops
   // at the first failure); the flag rows rotate like the bucket sets
   const unsigned halt = io.halt_in[r];
@@ -91,7 +92,7 @@ __global__ void __launch_bounds__(kStepMaxBlock) k_step_tiled(MeshDev mesh, cons
   const int i = tile.first + tid / LANES;
 
   int sgn = 0;
-  double pcs = 0.0, psn = 0.0;
+  long long pcs = 0, psn = 0;
   if (i < rep.n) {
     const int g = rep.off + i;
     const float4 me = io.xycs_in[g];
@@ -107,7 +108,7 @@ __global__ void __launch_bounds__(kStepMaxBlock) k_step_tiled(MeshDev mesh, cons
     } else {
       const size_t shift = io.noise_row0 ? (size_t)io.noise_row0[0] * io.noise_stride : 0;
       const float eta = io.noise ? io.noise[shift + g] : 0.0f;
-      const GlobalSource src{replica_in(bin, r, nv, rep.off), io.xycs_in, io.th_in};
+      const GlobalSource<OCC> src{replica_in(bin, r, nv, rep.off), io.xycs_in, io.th_in, s_occ};
       const StepOut o = particle_step<LANES>(mesh, rep.prm, src, g, me, th, v, eta, lane);
       if (lane == 0) {
         io.xycs_out[g] = make_float4(o.xo, o.yo, o.co, o.so);
@@ -130,12 +131,13 @@ __global__ void __launch_bounds__(kStepMaxBlock) k_step_tiled(MeshDev mesh, cons
           io.tape_q[g] = o.q;
         }
         sgn = o.sgn;
-        pcs = (double)o.co;
-        psn = (double)o.so;
+        pcs = __double2ll_rn((double)o.co * kPhiScale);
+        psn = __double2ll_rn((double)o.so * kPhiScale);
       }
     }
   }
-  block_order_partial(sgn, pcs, psn, io.partial + tile.logical);
+  const int wpb = (int)blockDim.x >> 6;
+  wave_order_partial(sgn, pcs, psn, io.partial + (size_t)tile.logical * wpb + (tid >> 6));
 }
 
 // crowded buckets up to this many particles are sorted in LDS (larger ones in global memory)
@@ -263,23 +265,42 @@ __global__ void __launch_bounds__(kStepMaxBlock) k_bucket_finalize(const Replica
   }
 }
 
-__global__ void k_phi_tiles(int R, int steps, const int* tile_begin, int n_logical, const OrderPartial* partial,
-                            const ReplicaDesc* reps, double* phi, int phi_stride, int t_base) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= R * steps) return;
-  const int r = idx / steps, s = idx % steps;
-  double cs = 0.0, sn = 0.0;
+// order parameters of replica r at step s, one workgroup per (r, s): the threads add the
+// replica's wave partials of the step in a strided split, then a wave and a workgroup sum (integer,
+// so the split does not matter)
+constexpr int kPhiBlock = 256;
+__global__ void __launch_bounds__(kPhiBlock) k_phi_tiles(int steps, const int* tile_begin, int n_logical, int per_tile,
+                                                         const OrderPartial* partial, const ReplicaDesc* reps,
+                                                         double* phi, int phi_stride, int t_base) {
+  __shared__ long long s_cs[kPhiBlock / 64], s_sn[kPhiBlock / 64];
+  __shared__ long s_sg[kPhiBlock / 64];
+  const int r = blockIdx.x / steps, s = blockIdx.x % steps;
+  long long cs = 0, sn = 0;
   long sg = 0;
-  for (int b = tile_begin[r]; b < tile_begin[r + 1]; ++b) {
-    const OrderPartial p = partial[(size_t)s * n_logical + b];
+  const OrderPartial* row = partial + (size_t)s * n_logical * per_tile;
+  for (int b = tile_begin[r] * per_tile + (int)threadIdx.x; b < tile_begin[r + 1] * per_tile; b += kPhiBlock) {
+    const OrderPartial p = row[b];
     cs += p.cs;
     sn += p.sn;
     sg += p.sgn;
   }
-  const int n = reps[r].n;
-  double* out = phi + 2 * ((size_t)r * phi_stride + t_base + s);
-  out[0] = (1.0 / n) * fabs((double)sg);
-  out[1] = sqrt(cs * cs + sn * sn) / n;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    cs += __shfl_xor(cs, off, 64);
+    sn += __shfl_xor(sn, off, 64);
+    sg += __shfl_xor(sg, off, 64);
+  }
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { s_cs[wave] = cs; s_sn[wave] = sn; s_sg[wave] = sg; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kPhiBlock / 64; ++w) { cs += s_cs[w]; sn += s_sn[w]; sg += s_sg[w]; }
+    const int n = reps[r].n;
+    const double c = (double)cs / kPhiScale, q = (double)sn / kPhiScale;
+    double* out = phi + 2 * ((size_t)r * phi_stride + t_base + s);
+    out[0] = (1.0 / n) * fabs((double)sg);
+    out[1] = sqrt(c * c + q * q) / n;
+  }
 }
 
 __global__ void k_flags_begin(int R, const unsigned* halt, unsigned* rows, int* first_bad) {
@@ -397,16 +418,27 @@ __global__ void k_neighbor_lists(MeshDev mesh, const ReplicaDesc* reps, const in
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
+template <int LANES, bool OCC>
+void launch_tiled_variant(int block, bool tape, const MeshDev& mesh, const ReplicaDesc* reps, const TileDesc* tiles,
+                          int n_tiles, const BucketSet& bin, const BucketSet& bout, const BucketSet& bclr,
+                          const TiledIO& io, hipStream_t st) {
+  const size_t lds = OCC ? sizeof(unsigned) * (size_t)occ_words(mesh.nv) : 0;
+  if (tape)
+    hipLaunchKernelGGL((k_step_tiled<LANES, true, OCC>), dim3(n_tiles), dim3(block), lds, st, mesh, reps, tiles, bin,
+                       bout, bclr, io);
+  else
+    hipLaunchKernelGGL((k_step_tiled<LANES, false, OCC>), dim3(n_tiles), dim3(block), lds, st, mesh, reps, tiles,
+                       bin, bout, bclr, io);
+}
+
 template <int LANES>
 void launch_tiled_lanes(int block, bool tape, const MeshDev& mesh, const ReplicaDesc* reps, const TileDesc* tiles,
                         int n_tiles, const BucketSet& bin, const BucketSet& bout, const BucketSet& bclr,
                         const TiledIO& io, hipStream_t st) {
-  if (tape)
-    hipLaunchKernelGGL((k_step_tiled<LANES, true>), dim3(n_tiles), dim3(block), 0, st, mesh, reps, tiles, bin, bout,
-                       bclr, io);
+  if (stage_occupancy(mesh.nv, block, LANES))
+    launch_tiled_variant<LANES, true>(block, tape, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st);
   else
-    hipLaunchKernelGGL((k_step_tiled<LANES, false>), dim3(n_tiles), dim3(block), 0, st, mesh, reps, tiles, bin,
-                       bout, bclr, io);
+    launch_tiled_variant<LANES, false>(block, tape, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st);
 }
 
 }  // namespace
@@ -432,9 +464,10 @@ hipError_t launch_bucket_finalize(int block, const ReplicaDesc* reps, const Tile
   return hipGetLastError();
 }
 
-hipError_t launch_phi_tiles(int R, int steps, const int* tile_begin, int n_logical, const OrderPartial* partial,
-                            const ReplicaDesc* reps, double* phi, int phi_stride, int t_base, hipStream_t st) {
-  hipLaunchKernelGGL(k_phi_tiles, dim3(cdiv((long)R * steps, 64)), dim3(64), 0, st, R, steps, tile_begin, n_logical,
+hipError_t launch_phi_tiles(int R, int steps, const int* tile_begin, int n_logical, int per_tile,
+                            const OrderPartial* partial, const ReplicaDesc* reps, double* phi, int phi_stride,
+                            int t_base, hipStream_t st) {
+  hipLaunchKernelGGL(k_phi_tiles, dim3(R * steps), dim3(kPhiBlock), 0, st, steps, tile_begin, n_logical, per_tile,
                      partial, reps, phi, phi_stride, t_base);
   return hipGetLastError();
 }

@@ -7,9 +7,15 @@
 
 namespace tissue {
 
+// fixed-point scale of the order-parameter sums: cos and sin of the headings are summed as
+// integers llrint(value * 2^40), which is exact and associative, so the sums do not depend on the
+// order of the particles or on how they are grouped; each term is rounded by at most 2^-41, and
+// up to 2^22 particles per replica stay inside int64
+constexpr double kPhiScale = 1099511627776.0;  // 2^40
+
 struct OrderPartial {
-  double cs, sn;   // sums of cos / sin of the new headings
-  int sgn, pad;    // sum of sign(x vy - y vx)
+  long long cs, sn;  // sums of cos / sin of the new headings, fixed point (kPhiScale)
+  int sgn, pad;      // sum of sign(x vy - y vx)
 };
 
 // one replica: particles [off, off + n) of the concatenated arrays
@@ -23,14 +29,16 @@ struct TileDesc {
   int r;        // replica; -1 marks an idle workgroup (padding of the XCD-balanced order)
   int first;    // first particle of the tile, counted within the replica
   int q, nq;    // tile index within the replica, number of tiles of the replica
-  int logical;  // slot of the tile's order-parameter partial (tiles of a replica are consecutive)
+  int logical;  // the tile's order-parameter partials start at logical * partials_per_tile(block)
+                // (tiles of a replica are consecutive)
   int pad;
 };
 
 // one vertex-bucket set of all replicas: replica r owns count[r*nv ..], slot[r*nv*kSlots ..],
-// the counters ovf_n[r], multi_n[r], big_n[r] and the entries ovf, multi and big at
-// [off_r .. off_r + n_r)
+// the occupancy mask occ[r*occ_words(nv) ..], the counters ovf_n[r], multi_n[r], big_n[r] and
+// the entries ovf, multi and big at [off_r .. off_r + n_r)
 struct BucketSet {
+  unsigned* occ;
   int* count;
   int* slot;
   int* ovf_n;
@@ -66,7 +74,7 @@ struct TiledIO {
   const float* noise;       // eta of this step, [ntot] (may be null: zero noise), shifted by
   const int* noise_row0;    //   noise_row0[0] rows of noise_stride (null: no shift); lets one
   size_t noise_stride;      //   captured graph serve any first noise row
-  OrderPartial* partial;    // [logical tiles] of this step
+  OrderPartial* partial;    // [logical tiles][partials_per_tile] of this step
   const unsigned* halt_in;  // [R] flags raised before this step (a flagged replica passes through)
   unsigned* halt_out;       // [R] flags raised up to and including this step
   unsigned* halt_clr;       // [R] the third flag row, cleared for the next step
@@ -104,6 +112,16 @@ struct ResidentIO {
 // largest workgroup of the tiled step kernel and of the persistent kernel (register budget of
 // two waves per SIMD)
 constexpr int kStepMaxBlock = 512;
+
+// Whether the tiled step kernel stages the replica's vertex-occupancy mask in LDS: each workgroup
+// loads the whole mask (one word per 32 vertices) before its first particle, which pays when the
+// workgroup holds enough particles per word (measured on MI355X: 256 replicas, 64 particles per
+// workgroup and 324 words, 7 % faster; 2,000 particles at 8 per workgroup, 6 % slower; 100,000
+// particles at 64 per workgroup and 3,134 words, 1 % slower). Results are the same either way.
+constexpr int kOccWordsPerParticle = 8;
+inline bool stage_occupancy(int nv, int block, int lanes) {
+  return tdev::occ_words(nv) <= kOccWordsPerParticle * (block / lanes);
+}
 constexpr int kResidentMaxBlock = 512;
 
 // lanes per particle for the persistent kernel when not tuned: wider groups for smaller replicas
@@ -148,10 +166,14 @@ hipError_t launch_step_tiled(int lanes, int block, bool tape, const tdev::MeshDe
 // count when there is one replica (its tiles are then not read), else 0
 hipError_t launch_bucket_finalize(int block, const ReplicaDesc* reps, const TileDesc* tiles, int n_tiles, int nv,
                                   int single_n, const BucketSet& bs, hipStream_t st);
-// order parameters of `steps` steps of every replica from the tiles' partials
-// (partial[s * n_logical + tile]); replica r's tiles are tile_begin[r] .. tile_begin[r+1]
-hipError_t launch_phi_tiles(int R, int steps, const int* tile_begin, int n_logical, const OrderPartial* partial,
-                            const ReplicaDesc* reps, double* phi, int phi_stride, int t_base, hipStream_t st);
+// order parameters of `steps` steps of every replica from the step kernel's partials, one per
+// wave: per_tile = waves per workgroup, partial[(s * n_logical + tile) * per_tile + wave];
+// replica r's tiles are tile_begin[r] .. tile_begin[r+1]
+hipError_t launch_phi_tiles(int R, int steps, const int* tile_begin, int n_logical, int per_tile,
+                            const OrderPartial* partial, const ReplicaDesc* reps, double* phi, int phi_stride,
+                            int t_base, hipStream_t st);
+// order-parameter partials the tiled step kernel writes per tile (one per wave)
+inline int partials_per_tile(int block) { return block / 64; }
 // run start: rows[0] = halt, rows[1] = 0, first_bad = INT_MAX; run end: halt = last row
 hipError_t launch_flags_begin(int R, const unsigned* halt, unsigned* rows, int* first_bad, hipStream_t st);
 hipError_t launch_flags_end(int R, const unsigned* last_row, unsigned* halt, hipStream_t st);

@@ -40,8 +40,10 @@ struct Layout {
   static constexpr size_t xycs_out = xycs_in + al(sizeof(float4) * kMaxParticles);
   static constexpr size_t vid = xycs_out + al(sizeof(float4) * kMaxParticles);
   static constexpr size_t rid = vid + al(sizeof(int) * kMaxParticles);
-  // one bucket set: count[V], slot[V * kSlots], ovf[N] (int2), multi[N], big[N], counters[3]
-  static constexpr size_t set_count = 0;
+  // one bucket set: occupancy mask, count[V], slot[V * kSlots], ovf[N] (int2), multi[N], big[N],
+  // counters[3]
+  static constexpr size_t set_occ = 0;
+  static constexpr size_t set_count = set_occ + al(sizeof(unsigned) * tdev::occ_words(kMaxVertices));
   static constexpr size_t set_slot = set_count + al(sizeof(int) * kMaxVertices);
   static constexpr size_t set_ovf = set_slot + al(sizeof(int) * kMaxVertices * tdev::kSlots);
   static constexpr size_t set_multi = set_ovf + al(sizeof(int2) * kMaxParticles);
@@ -50,7 +52,7 @@ struct Layout {
   static constexpr size_t set_bytes = set_ctr + al(sizeof(int) * 3);
   static constexpr size_t sets = rid + al(sizeof(int) * kMaxParticles);
   static constexpr size_t partial = sets + 3 * set_bytes;
-  static constexpr size_t tiles = partial + al(sizeof(OrderPartial) * kMaxTiles);
+  static constexpr size_t tiles = partial + al(sizeof(OrderPartial) * kMaxTiles * (tissue::kStepMaxBlock / 64));
   static constexpr size_t rep = tiles + al(sizeof(TileDesc) * kMaxTiles);
   static constexpr size_t tile_begin = rep + al(sizeof(ReplicaDesc));
   static constexpr size_t rows = tile_begin + al(sizeof(int) * 2);
@@ -83,6 +85,7 @@ BucketSet set_at(unsigned char* base, int k, int sorted) {
   unsigned char* s = base + Layout::sets + (size_t)k * Layout::set_bytes;
   int* ctr = (int*)(s + Layout::set_ctr);
   BucketSet b;
+  b.occ = (unsigned*)(s + Layout::set_occ);
   b.count = (int*)(s + Layout::set_count);
   b.slot = (int*)(s + Layout::set_slot);
   b.ovf = (int2*)(s + Layout::set_ovf);
@@ -105,6 +108,7 @@ struct SetupArgs {
   int* vid;
   int* rid;
   int* counts[3];
+  unsigned* occ[3];
   int* ctrs[3];
   TileDesc* tiles;
   ReplicaDesc* rep;
@@ -117,7 +121,8 @@ struct SetupArgs {
 
 // packs the state (x, y, cos theta, sin theta), copies the vertex ids (an id outside the mesh is
 // replaced by 0 and reported with TISSUE_FLAG_BAD_INPUT, so no later access leaves its table),
-// clears the bucket sets and the flag rows, and writes the tile and replica descriptors
+// clears the bucket sets (counts, occupancy masks, list counters) and the flag rows, and writes
+// the tile and replica descriptors
 __global__ void k_noalloc_setup(SetupArgs a) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < a.n) {
@@ -134,6 +139,11 @@ __global__ void k_noalloc_setup(SetupArgs a) {
     a.counts[0][t] = 0;
     a.counts[1][t] = 0;
     a.counts[2][t] = 0;
+  }
+  if (t < tdev::occ_words(a.nv)) {
+    a.occ[0][t] = 0u;
+    a.occ[1][t] = 0u;
+    a.occ[2][t] = 0u;
   }
   if (t < a.n_tiles) a.tiles[t] = TileDesc{0, t * a.per_tile, t, a.n_tiles, t, 0};
   if (t < 3) {
@@ -220,6 +230,7 @@ extern "C" int32_t tissue_step_noalloc(const tissue_mesh* mc, const tissue_param
     a.rid = (int*)(base + Layout::rid);
     for (int k = 0; k < 3; ++k) {
       a.counts[k] = b[k].count;
+      a.occ[k] = b[k].occ;
       a.ctrs[k] = b[k].ovf_n;
     }
     a.tiles = (TileDesc*)(base + Layout::tiles);
@@ -263,7 +274,8 @@ extern "C" int32_t tissue_step_noalloc(const tissue_mesh* mc, const tissue_param
     io.first_bad = a.first_bad;
     io.step = 0;
     HIP_OK(tissue::launch_step_tiled(lanes, block, false, md, a.rep, a.tiles, n_tiles, b[0], b[1], b[2], io, st));
-    HIP_OK(tissue::launch_phi_tiles(1, 1, a.tile_begin, n_tiles, io.partial, a.rep, phi, 1, 0, st));
+    HIP_OK(tissue::launch_phi_tiles(1, 1, a.tile_begin, n_tiles, tissue::partials_per_tile(block), io.partial, a.rep,
+                                    phi, 1, 0, st));
     hipLaunchKernelGGL(k_noalloc_finish, dim3(cdiv(n, 256)), dim3(256), 0, st, n, io.xycs_out, x_out, y_out,
                        io.halt_out, flags);
     HIP_OK(hipGetLastError());
