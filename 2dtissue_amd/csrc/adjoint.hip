@@ -310,6 +310,8 @@ hipError_t launch_adj_gather(int lanes, const MeshDev& mesh, const ReplicaDesc* 
     case 4: TISSUE_ADJ_GATHER(4); break;
     case 8: TISSUE_ADJ_GATHER(8); break;
     case 16: TISSUE_ADJ_GATHER(16); break;
+    case 32: TISSUE_ADJ_GATHER(32); break;
+    case 64: TISSUE_ADJ_GATHER(64); break;
     default: return hipErrorInvalidValue;
   }
 #undef TISSUE_ADJ_GATHER

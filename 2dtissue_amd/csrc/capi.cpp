@@ -190,8 +190,7 @@ int32_t tissue_seam_wrap(int32_t n, const double* start_xy, double* end_xy, doub
                          int32_t max_crossings, int32_t lanes) {
   return guarded([&] {
     if (n < 0 || (n > 0 && (!start_xy || !end_xy || !dtheta))) throw ApiError(TISSUE_E_INVALID_ARG, "bad seam-wrap arguments");
-    if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16)
-      throw ApiError(TISSUE_E_INVALID_ARG, "lanes must be 1, 2, 4, 8 or 16");
+    if (!valid_lanes(lanes)) throw ApiError(TISSUE_E_INVALID_ARG, "lanes must be 1, 2, 4, 8, 16, 32 or 64");
     if ((int64_t)n * lanes > INT_MAX - 256) throw ApiError(TISSUE_E_INVALID_ARG, "too many moves for one call");
     for (int64_t q = 0; q < 2 * (int64_t)n; ++q)
       if (!std::isfinite(start_xy[q]) || !std::isfinite(end_xy[q])) throw ApiError(TISSUE_E_INVALID_ARG, "non-finite position");

@@ -113,8 +113,8 @@ int32_t tissue_batch_get_flags(tissue_batch* b, uint32_t* flags, int32_t* first_
 int32_t tissue_batch_set_launch(tissue_batch* b, int32_t lanes, int32_t block, int32_t steps_per_launch) {
   return guarded([&] {
     auto& e = eng(b);
-    if (lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16)
-      throw ApiError(TISSUE_E_INVALID_ARG, "lanes must be 0, 1, 2, 4, 8 or 16");
+    if (lanes != 0 && !valid_lanes(lanes))
+      throw ApiError(TISSUE_E_INVALID_ARG, "lanes must be 0, 1, 2, 4, 8, 16, 32 or 64");
     if (block != 0 && (block < 64 || block > tissue::kStepMaxBlock || block % 64))
       throw ApiError(TISSUE_E_INVALID_ARG, "block must be a multiple of 64 in [64, 512]");
     if (steps_per_launch < 0) throw ApiError(TISSUE_E_INVALID_ARG, "negative steps per launch");

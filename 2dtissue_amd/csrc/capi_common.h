@@ -79,6 +79,9 @@ inline float float_below(double T, bool strict) {
   return f;
 }
 
+// lanes per particle the kernels are built for
+inline bool valid_lanes(int l) { return l == 1 || l == 2 || l == 4 || l == 8 || l == 16 || l == 32 || l == 64; }
+
 // the reference's tiling loop has no bound; this one only stops pathological inputs
 constexpr int kDefaultMaxCrossings = 16384;
 

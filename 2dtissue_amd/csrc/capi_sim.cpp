@@ -149,8 +149,8 @@ int32_t tissue_neighbor_lists(tissue_sim* s, int32_t* counts, int32_t* cols, int
 int32_t tissue_set_launch(tissue_sim* s, int32_t lanes, int32_t block) {
   return guarded([&] {
     auto& e = eng(s);
-    if (lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16)
-      throw ApiError(TISSUE_E_INVALID_ARG, "lanes must be 0, 1, 2, 4, 8 or 16");
+    if (lanes != 0 && !valid_lanes(lanes))
+      throw ApiError(TISSUE_E_INVALID_ARG, "lanes must be 0, 1, 2, 4, 8, 16, 32 or 64");
     if (block != 0 && (block < 64 || block > tissue::kStepMaxBlock || block % 64))
       throw ApiError(TISSUE_E_INVALID_ARG, "block must be a multiple of 64 in [64, 512]");
     tissue::Shape r = e.req;

@@ -635,14 +635,17 @@ Shape Engine::autotune(int n_steps) {
   if (rec) throw ApiError(TISSUE_E_INVALID_ARG, "stop the output stream before autotuning");
   if (n_steps < 1) throw ApiError(TISSUE_E_INVALID_ARG, "autotune needs at least one step");
   std::vector<Shape> cands;
-  for (int l : {1, 2, 4, 8})
-    for (int b : {128, 256, 512})
+  for (int l : {1, 2, 4, 8, 16, 32, 64})
+    for (int b : {128, 256, 512}) {
+      if (l >= 16 && ntot > 16384) continue;  // wide groups only pay for small systems
+      if (b < l) continue;
       for (int x : {0, 1}) {
         if (x == 1 && R < kXcds) continue;
         Shape s;
         s.mode = TISSUE_MODE_STEP_KERNEL; s.lanes = l; s.block = b; s.spl = kDefaultSpl; s.xcd = x;
         cands.push_back(s);
       }
+    }
   if (nmax <= resident_cap)
     for (int l : {1, 2, 4, 8})
       for (int b : {256, 512}) {

@@ -453,6 +453,8 @@ hipError_t launch_step_tiled(int lanes, int block, bool tape, const MeshDev& mes
     case 4: launch_tiled_lanes<4>(block, tape, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st); break;
     case 8: launch_tiled_lanes<8>(block, tape, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st); break;
     case 16: launch_tiled_lanes<16>(block, tape, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st); break;
+    case 32: launch_tiled_lanes<32>(block, tape, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st); break;
+    case 64: launch_tiled_lanes<64>(block, tape, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -503,6 +505,8 @@ hipError_t launch_seam_wrap(int lanes, int n, const double2* start, double2* end
     case 4: hipLaunchKernelGGL(k_seam_wrap<4>, grid, block, 0, st, n, start, end, dtheta, quarter, ok, max_crossings); break;
     case 8: hipLaunchKernelGGL(k_seam_wrap<8>, grid, block, 0, st, n, start, end, dtheta, quarter, ok, max_crossings); break;
     case 16: hipLaunchKernelGGL(k_seam_wrap<16>, grid, block, 0, st, n, start, end, dtheta, quarter, ok, max_crossings); break;
+    case 32: hipLaunchKernelGGL(k_seam_wrap<32>, grid, block, 0, st, n, start, end, dtheta, quarter, ok, max_crossings); break;
+    case 64: hipLaunchKernelGGL(k_seam_wrap<64>, grid, block, 0, st, n, start, end, dtheta, quarter, ok, max_crossings); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -115,7 +115,7 @@ int32_t tissue_get_flags(tissue_sim* s, uint32_t* flags, int32_t* first_step);
 /* neighbour sets {j : d_ij < 2 sigma} (self included) of the current state, at most cap per
  * particle: counts[n], cols[n*cap], each list ascending */
 int32_t tissue_neighbor_lists(tissue_sim* s, int32_t* counts, int32_t* cols, int32_t cap);
-/* launch shape: lanes per particle (1, 2, 4, 8, 16) and threads per block (64..512); 0 = auto */
+/* launch shape: lanes per particle (1, 2, 4, 8, 16, 32, 64) and threads per block (64..512); 0 = auto */
 int32_t tissue_set_launch(tissue_sim* s, int32_t lanes, int32_t block);
 /* execution mode of tissue_step*: one kernel launch per step over the whole GPU, or the
  * persistent LDS-resident kernel (one workgroup, many steps per launch; needs
@@ -160,7 +160,7 @@ int32_t tissue_batch_run(tissue_batch* b, int32_t n_steps, int32_t first_row);
 int32_t tissue_batch_get_order(tissue_batch* b, int32_t replica, double* phi_ref, double* phi_polar, int32_t cap,
                                int32_t* count);
 int32_t tissue_batch_get_flags(tissue_batch* b, uint32_t* flags /* [n_replicas] */, int32_t* first_step);
-/* lanes per particle (1, 2, 4, 8, 16; the persistent kernel uses at most 8), threads per
+/* lanes per particle (1, 2, 4, 8, 16, 32, 64; the persistent kernel uses at most 8), threads per
  * workgroup (64..512), steps per launch; 0 = auto */
 int32_t tissue_batch_set_launch(tissue_batch* b, int32_t lanes, int32_t block, int32_t steps_per_launch);
 /* TISSUE_MODE_* and the tiled mode's workgroup order: xcd_order 1 keeps each replica on one XCD,

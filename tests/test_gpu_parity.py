@@ -40,7 +40,7 @@ def run_teacher_forced(T, O, cfg, steps, lanes=0, mode=0, sparse=False):
 
 
 @pytest.mark.parametrize("mode", [1, 2])
-@pytest.mark.parametrize("lanes", [1, 8])
+@pytest.mark.parametrize("lanes", [1, 8, 32, 64])
 def test_c1_teacher_forced_50_steps(T, O, lanes, mode):
     total = run_teacher_forced(T, O, P.C1, 50, lanes, mode)
     assert all(v == 0 for v in total.values()), total
