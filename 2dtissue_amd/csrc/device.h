@@ -48,8 +48,9 @@ struct BucketsIn {
 };
 // The same buckets while the step fills them: the first kSlots arrivals of a vertex take its
 // slots, later ones go to the (vertex, id) overflow list; the second arrival at a vertex also
-// lists the vertex in multi (the buckets the finalize pass must sort); the first arrival sets the
-// vertex's bit in the occupancy mask occ (bit v % 32 of word v / 32).
+// lists the vertex in multi (the buckets the finalize pass must sort; null when no finalize pass
+// follows); the first arrival sets the vertex's bit in the occupancy mask occ (bit v % 32 of word
+// v / 32).
 struct BucketsOut { int* count; int* slot; int* ovf_n; int2* ovf; int* multi_n; int* multi; unsigned* occ; };
 
 // words of a vertex-occupancy mask over nv vertices

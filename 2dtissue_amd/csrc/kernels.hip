@@ -30,12 +30,12 @@ __device__ __forceinline__ void bucket_insert(const BucketsOut& b, int v, int i)
     const int o = atomicAdd(b.ovf_n, 1);
     b.ovf[o] = make_int2(v, i);
   }
-  if (old == 1) b.multi[atomicAdd(b.multi_n, 1)] = v;  // the bucket needs sorting
+  if (old == 1 && b.multi) b.multi[atomicAdd(b.multi_n, 1)] = v;  // the bucket needs sorting
 }
 
 __device__ __forceinline__ BucketsOut replica_out(const BucketSet& s, int r, int nv, int off) {
   return {s.count + (size_t)r * nv, s.slot + (size_t)r * nv * kSlots, s.ovf_n + r, s.ovf + off, s.multi_n + r,
-          s.multi + off, s.occ + (size_t)r * occ_words(nv)};
+          s.multi ? s.multi + off : nullptr, s.occ + (size_t)r * occ_words(nv)};
 }
 
 // orders the LDS / global accesses of one wave (its lanes exchange data through memory)

@@ -36,7 +36,8 @@ struct TileDesc {
 
 // one vertex-bucket set of all replicas: replica r owns count[r*nv ..], slot[r*nv*kSlots ..],
 // the occupancy mask occ[r*occ_words(nv) ..], the counters ovf_n[r], multi_n[r], big_n[r] and
-// the entries ovf, multi and big at [off_r .. off_r + n_r)
+// the entries ovf, multi and big at [off_r .. off_r + n_r). As the set a step fills, multi may be
+// null: no finalize pass follows, so the step lists no bucket for sorting.
 struct BucketSet {
   unsigned* occ;
   int* count;
