@@ -118,13 +118,19 @@ def valu_axis(ps: float, pairs_per_particle: float, faces_per_locate: float, iss
 
 
 def cpu_model() -> str:
-    """The host CPU's model name as `lscpu` reports it (falls back to /proc/cpuinfo)."""
+    """The host CPU's model name as `lscpu` reports it, with its family / model / stepping numbers
+    (a virtual machine may report only a generic name); falls back to /proc/cpuinfo."""
     import subprocess
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        f = {}
         for line in out.splitlines():
-            if line.strip().lower().startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = line.split(":", 1)
+                f.setdefault(k.strip().lower(), v.strip())
+        if "model name" in f:
+            ids = [f"{k} {f[k]}" for k in ("cpu family", "model", "stepping") if k in f]
+            return f["model name"] + (f" ({', '.join(ids)})" if ids else "")
     except (OSError, subprocess.SubprocessError):
         pass
     try:
