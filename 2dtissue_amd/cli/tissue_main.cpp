@@ -2,7 +2,7 @@
 // include/2DTissue.h. Flags and their mapping are the reference's: --step-time is passed as v0
 // (main.cpp:76-84). Additions: --sigma, --seed, --noise-std, --data-dir. Without --mesh-path the
 // mesh is the built-in ellipsoid (2, 1, 1) at icosphere frequency 22 (the reference's default
-// mesh lives in MeshCartographyLib, which this build does not have).
+// mesh lives in MeshCartographyLib, which is not part of this library).
 #include <chrono>
 #include <cstdlib>
 #include <cstring>

@@ -431,7 +431,7 @@ def main():
                     help="first run a device copy of MIB MiB (counter calibration for profiles)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (replicas would shard "
+        print("bench.py measures one GPU: the library has no multi-GPU path (replicas would shard "
               "across GPUs without per-step communication); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     calib = copy_calibration(args.copy_calibration) if args.copy_calibration > 0 else None

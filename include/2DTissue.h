@@ -67,7 +67,7 @@ class _2DTissue {
       : save_data_(save_data), particle_count_(particle_count), step_count_(step_count) {
     (void)particle_innenleben; (void)free_boundary; (void)k_next; (void)v0_next; (void)r_adh; (void)k_adh;
     (void)map_cache_count;
-    if (use_kafka) throw std::invalid_argument("Kafka streaming is not supported by this build");
+    if (use_kafka) throw std::invalid_argument("Kafka streaming is not supported by this library");
     if (particle_count < 1 || step_count < 0) throw std::invalid_argument("particle_count and step_count");
     try {
       check(open_mesh(mesh_path, &mesh_));
