@@ -638,7 +638,7 @@ Shape Engine::autotune(int n_steps) {
   if (n_steps < 1) throw ApiError(TISSUE_E_INVALID_ARG, "autotune needs at least one step");
   std::vector<Shape> cands;
   for (int l : {1, 2, 4, 8, 16, 32, 64})
-    for (int b : {128, 256, 512}) {
+    for (int b : {64, 128, 256, 512}) {
       if (l >= 16 && ntot > 16384) continue;  // wide groups only pay for small systems
       if (b < l) continue;
       for (int x : {0, 1}) {

@@ -130,13 +130,15 @@ inline int resident_default_lanes(int nmax) { return nmax <= 256 ? 8 : (nmax <= 
 
 // Default launch shape of the tiled step kernel (lanes per particle, threads per workgroup) for
 // R replicas of ntot particles in all. Measured on MI355X: a single system of a few thousand
-// particles fills few CUs, so more lanes per particle shorten its critical path (2,000 particles:
-// 16 lanes 25.2 us, 8 lanes 27.2 us); a single 100,000-particle system is fastest with 4 (350 us;
-// 2 lanes 383 us), a large batch with 2 (autotuned).
+// particles fills few CUs, so more lanes per particle shorten its critical path, and one-wave
+// workgroups let every wave start its walk without waiting for another at the staging barrier
+// (2,000 particles: 32 lanes in 64 threads 18.97 us, 16 in 128 19.30 us, 64 in 64 19.19 us, 8 in
+// 64 20.7 us); a single 100,000-particle system is fastest with 4 (2 lanes 10 % slower), a large
+// batch with 2 (autotuned).
 inline void tiled_default_shape(int R, int ntot, int* lanes, int* block) {
   const int big_lanes = R == 1 ? 4 : 2;
-  *lanes = ntot <= 4096 ? 16 : (ntot <= 8192 ? 8 : (ntot <= 65536 ? 4 : big_lanes));
-  *block = ntot <= 4096 ? 128 : 256;
+  *lanes = ntot <= 4096 ? 32 : (ntot <= 8192 ? 8 : (ntot <= 65536 ? 4 : big_lanes));
+  *block = ntot <= 4096 ? 64 : 256;
 }
 
 // Replicas of a few thousand particles, at most one per two vertices, leave their vertex buckets

@@ -28,9 +28,9 @@ using tissue::TiledIO;
 
 constexpr int kMaxParticles = TISSUE_NOALLOC_MAX_PARTICLES;
 constexpr int kMaxVertices = TISSUE_NOALLOC_MAX_UV_VERTICES;
-// tiles of the default launch shape: at most 8 particles per tile (16 lanes in 128 threads) up to
-// 4,096 particles, 32 up to 8,192 and 64 beyond, so at most 512 tiles
-constexpr int kMaxTiles = 512;
+// tiles of the default launch shape: 2 particles per tile (32 lanes in 64 threads) up to 4,096
+// particles, 32 up to 8,192 and 64 beyond, so at most 2,048 tiles
+constexpr int kMaxTiles = 2048;
 
 constexpr size_t al(size_t x) { return (x + 255) & ~(size_t)255; }
 

@@ -253,7 +253,7 @@ int32_t tissue_seam_wrap(int32_t n, const double* start_xy, double* end_xy, doub
  * rejected with TISSUE_E_INVALID_ARG. */
 #define TISSUE_NOALLOC_MAX_PARTICLES 16384
 #define TISSUE_NOALLOC_MAX_UV_VERTICES 32768
-#define TISSUE_NOALLOC_SCRATCH_BYTES (5u << 20)
+#define TISSUE_NOALLOC_SCRATCH_BYTES (6u << 20)
 int32_t tissue_step_noalloc(const tissue_mesh* mesh, const tissue_params* p, int32_t n, const float* x_in,
                             const float* y_in, const float* theta_in, const int32_t* vid_in, const float* noise,
                             float* x_out, float* y_out, float* theta_out, int32_t* vid_out, float* vx, float* vy,

@@ -139,7 +139,7 @@ def test_batch_autotune_restores_state(T):
     b.gen_noise(seeds, std, 10)
     before = b.get_state()
     lanes, block = b.autotune(10)
-    assert lanes in (1, 2, 4, 8, 16, 32, 64) and block in (128, 256, 512)
+    assert lanes in (1, 2, 4, 8, 16, 32, 64) and block in (64, 128, 256, 512)
     assert b.launch()["mode"] in (TILED, RESIDENT)
     assert same_state(before, b.get_state())
     assert len(b.order(0)[0]) == 0
