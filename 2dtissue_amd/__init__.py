@@ -134,6 +134,8 @@ SIGNATURES = [
     ("tissue_reset_order", C.c_int32, [_P]),
     ("tissue_get_flags", C.c_int32, [_P, C.POINTER(C.c_uint32), _I]),
     ("tissue_neighbor_lists", C.c_int32, [_P, _I, _I, C.c_int32]),
+    ("tissue_set_visit_check", C.c_int32, [_P, C.c_int32]),
+    ("tissue_get_visits", C.c_int32, [_P, _I, _I]),
     ("tissue_set_launch", C.c_int32, [_P, C.c_int32, C.c_int32]),
     ("tissue_synchronize", C.c_int32, [_P]),
     ("tissue_last_step_ms", C.c_int32, [_P, _D]),
@@ -379,6 +381,18 @@ class Sim:
         if counts.max(initial=0) > cap:
             return self.neighbor_lists(int(counts.max()))
         return [cols[i, : counts[i]].copy() for i in range(self.n)]
+
+    def set_visit_check(self, enable: bool = True) -> None:
+        """While on, each step of the step kernel records what its own neighbour walk visited
+        (tissue_set_visit_check)."""
+        _check(lib().tissue_set_visit_check(self.h, 1 if enable else 0))
+
+    def visits(self) -> tuple[np.ndarray, np.ndarray]:
+        """(na, hash) of the last step: the number of particles each particle's walk visited within
+        2 sigma (self included) and the sum mod 2^32 of visit_mix over their indices."""
+        na = np.zeros(self.n, np.int32); h = np.zeros(self.n, np.uint32)
+        _check(lib().tissue_get_visits(self.h, _ptr(na, C.c_int32), _ptr(h, C.c_int32)))
+        return na, h
 
     def set_launch(self, lanes: int = 0, block: int = 0) -> None:
         _check(lib().tissue_set_launch(self.h, lanes, block))

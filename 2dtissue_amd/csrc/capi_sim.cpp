@@ -146,6 +146,20 @@ int32_t tissue_neighbor_lists(tissue_sim* s, int32_t* counts, int32_t* cols, int
   });
 }
 
+int32_t tissue_set_visit_check(tissue_sim* s, int32_t enable) {
+  return guarded([&] {
+    eng(s).set_visit_check(enable != 0);
+    return TISSUE_OK;
+  });
+}
+
+int32_t tissue_get_visits(tissue_sim* s, int32_t* na, uint32_t* hash) {
+  return guarded([&] {
+    eng(s).get_visits(na, hash);
+    return TISSUE_OK;
+  });
+}
+
 int32_t tissue_set_launch(tissue_sim* s, int32_t lanes, int32_t block) {
   return guarded([&] {
     auto& e = eng(s);

@@ -126,7 +126,10 @@ struct StepOut {
   int nb;                    // neighbour count
   unsigned fl;               // validation flags
   int sgn;                   // sign of x vy - y vx (order parameter)
-  // what the adjoint replays (dead, and removed by the compiler, in kernels that do not record)
+  // what the adjoint replays and the visit check records (dead, and removed by the compiler, in
+  // kernels that do not record them)
+  int na;                    // particles visited within 2 sigma (the neighbour set, self included)
+  unsigned vh;               // sum of visit_mix(j) over them (mod 2^32): the set, order-free
   float Fx, Fy;              // force sum
   float Sx, Sy;              // alignment sums (cos, sin) over the neighbour set incl. self
   int q;                     // seam wrap: counter-clockwise quarter turns of the composed map
@@ -138,6 +141,17 @@ constexpr int kScanChunk = 8;  // measured on MI355X: 4 and 16 are slower
 // lanes per particle from which the walk loads bucket heads in its scan (measured on MI355X: 1.7 %
 // faster at 16 lanes, 1 % slower at 4 lanes with chunks of 4), and its chunk of entries per lane
 constexpr int kHeadScanLanes = 8;
+
+// 32-bit mixing of a particle index (the MurmurHash3 finaliser): summed over a neighbour set it
+// names the set independently of the visiting order
+__host__ __device__ inline unsigned visit_mix(unsigned h) {
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
 constexpr int kHeadChunk = 2;
 static_assert(32 % kScanChunk == 0, "scan chunks must tile the 32-entry window");
 
@@ -240,6 +254,7 @@ __device__ __forceinline__ StepOut particle_step(const MeshDev& mesh, const Step
   double Fx64 = 0.0, Fy64 = 0.0;  // the force, summed in FP64 (its FP32 rounding drives the FP32 path)
   const double kc64 = prm.k64 / prm.two_sigma64;  // Fij / d = kc (d - 2 sigma) / d, to a few ulps
   int na = 0, nb = 0;
+  unsigned vh = 0u;
   const int rs = mesh.row_ptr[v], re = mesh.row_ptr[v + 1];
   // pair set-up of a near entry (vertex w at distance d, bucket head h) and its particles
   auto visit = [&](int w, float d, const typename Src::Head& h) __attribute__((always_inline)) {
@@ -255,6 +270,7 @@ __device__ __forceinline__ StepOut particle_step(const MeshDev& mesh, const Step
       Sx += q.z;
       Sy += q.w;
       ++na;
+      vh += visit_mix((unsigned)j);
       if (counted) ++nb;
     });
   };
@@ -354,6 +370,7 @@ __device__ __forceinline__ StepOut particle_step(const MeshDev& mesh, const Step
   Sx = group_sum<LANES>(Sx);
   Sy = group_sum<LANES>(Sy);
   na = group_sum<LANES>(na);
+  vh = group_sum<LANES>(vh);
   nb = group_sum<LANES>(nb);
 
   // speed, velocity, Euler step (Locomotion.cpp:71-84)
@@ -446,6 +463,8 @@ __device__ __forceinline__ StepOut particle_step(const MeshDev& mesh, const Step
   o.vx = vx;
   o.vy = vy;
   o.nb = nb;
+  o.na = na;
+  o.vh = vh;
   // UV -> 3D remap of the stored FP32 position (CellHelper.cpp:71-85)
   o.L = locate<LANES>(mesh, (double)o.xo, (double)o.yo, lane);
   sincosf(o.to, &o.so, &o.co);

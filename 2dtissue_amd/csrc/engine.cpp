@@ -123,7 +123,7 @@ void Engine::release() {
     rec.reset();
   }
   drop_graphs();
-  dfree(d_reps); dfree(d_rid);
+  dfree(d_reps); dfree(d_rid); dfree(d_visit);
   for (int k = 0; k < 2; ++k) { dfree(xycs[k]); dfree(th[k]); dfree(vid[k]); }
   dfree(vx); dfree(vy); dfree(r3x); dfree(r3y); dfree(r3z); dfree(face); dfree(nbr);
   dfree(d_halt); dfree(d_rows); dfree(d_first_bad);
@@ -493,6 +493,7 @@ void Engine::enqueue(const Shape& s, int n_steps, const float* nz, size_t stride
         io.halt_clr = d_rows + (size_t)((t + 2) % 3) * R;
         io.first_bad = d_first_bad;
         io.step = t;
+        io.visit = d_visit;
         BucketSet out = bs[(bp + 1) % 3];
         if (raw_buckets) out.multi = nullptr;  // no finalize pass reads the sort list
         HIP_OK(launch_step_tiled(s.lanes, s.block, false, md, d_reps, d_tiles, n_phys, bs[bp], out, bs[(bp + 2) % 3],
@@ -718,6 +719,31 @@ Shape Engine::autotune(int n_steps) {
 }
 
 // ------------------------------------------------------------------ neighbour lists
+
+void Engine::set_visit_check(bool on) {
+  if (on == (d_visit != nullptr)) return;
+  HIP_OK(hipStreamSynchronize(stream));
+  drop_graphs();  // the captured steps launch the other kernel variant
+  if (on) {
+    d_visit = dalloc<int2>(ntot);
+    HIP_OK(hipMemsetAsync(d_visit, 0xff, ntot * sizeof(int2), stream));  // (-1, -1): no step recorded yet
+  } else {
+    dfree(d_visit);
+    d_visit = nullptr;
+  }
+}
+
+void Engine::get_visits(int32_t* na, uint32_t* hash) {
+  if (!na || !hash) throw ApiError(TISSUE_E_INVALID_ARG, "bad visit buffers");
+  if (!d_visit) throw ApiError(TISSUE_E_INVALID_ARG, "the visit check is off (tissue_set_visit_check)");
+  std::vector<int2> h(ntot);
+  HIP_OK(hipMemcpyAsync(h.data(), d_visit, ntot * sizeof(int2), hipMemcpyDeviceToHost, stream));
+  HIP_OK(hipStreamSynchronize(stream));
+  for (int i = 0; i < ntot; ++i) {
+    na[i] = h[i].x;
+    hash[i] = (uint32_t)h[i].y;
+  }
+}
 
 void Engine::neighbor_lists(int32_t* counts, int32_t* cols, int32_t cap) {
   if (!counts || !cols || cap < 1) throw ApiError(TISSUE_E_INVALID_ARG, "bad neighbour-list buffers");

@@ -38,6 +38,10 @@ struct Engine {
   void get_outputs(float* vx, float* vy, float* r3d, int32_t* face, int32_t* nbr);
   // neighbour sets {j : d_ij < 2 sigma} of the current state, ids counted within each replica
   void neighbor_lists(int32_t* counts, int32_t* cols, int32_t cap);
+  // visit check: while on, every step of the tiled kernel records, per particle, the size of the
+  // neighbour set its own walk visited and the sum of visit_mix over the visited (global) ids
+  void set_visit_check(bool on);
+  void get_visits(int32_t* na, uint32_t* hash);
 
   // noise: rows [rows][ntot] kept on the device for later runs
   void stage_noise(const float* h, int rows);
@@ -96,6 +100,7 @@ struct Engine {
 
   ReplicaDesc* d_reps = nullptr;
   int* d_rid = nullptr;  // replica of each particle
+  int2* d_visit = nullptr;  // visit check records (set_visit_check), null while off
   float4* xycs[2] = {nullptr, nullptr};
   float* th[2] = {nullptr, nullptr};
   int* vid[2] = {nullptr, nullptr};

@@ -57,7 +57,7 @@ __device__ __forceinline__ void wave_order_partial(int sgn, long long cs, long l
   if ((threadIdx.x & 63) == 0) *out = OrderPartial{cs, sn, sgn, 0};
 }
 
-template <int LANES, bool TAPE, bool OCC>
+template <int LANES, bool TAPE, bool OCC, bool VISITS>
 __global__ void __launch_bounds__(kStepMaxBlock) k_step_tiled(MeshDev mesh, const ReplicaDesc* reps,
                                                               const TileDesc* tiles, BucketSet bin, BucketSet bout,
                                                               BucketSet bclr, TiledIO io) {
@@ -104,6 +104,7 @@ __global__ void __launch_bounds__(kStepMaxBlock) k_step_tiled(MeshDev mesh, cons
         io.th_out[g] = th;
         io.vid_out[g] = v;
         bucket_insert(bo, v, g);
+        if constexpr (VISITS) io.visit[g] = make_int2(-1, 0);
       }
     } else {
       const size_t shift = io.noise_row0 ? (size_t)io.noise_row0[0] * io.noise_stride : 0;
@@ -130,6 +131,7 @@ __global__ void __launch_bounds__(kStepMaxBlock) k_step_tiled(MeshDev mesh, cons
           io.tape_fs[g] = make_float4(o.Fx, o.Fy, o.Sx, o.Sy);
           io.tape_q[g] = o.q;
         }
+        if constexpr (VISITS) io.visit[g] = make_int2(o.na, (int)o.vh);
         sgn = o.sgn;
         pcs = __double2ll_rn((double)o.co * kPhiScale);
         psn = __double2ll_rn((double)o.so * kPhiScale);
@@ -424,11 +426,14 @@ void launch_tiled_variant(int block, bool tape, const MeshDev& mesh, const Repli
                           const TiledIO& io, hipStream_t st) {
   const size_t lds = OCC ? sizeof(unsigned) * (size_t)occ_words(mesh.nv) : 0;
   if (tape)
-    hipLaunchKernelGGL((k_step_tiled<LANES, true, OCC>), dim3(n_tiles), dim3(block), lds, st, mesh, reps, tiles, bin,
-                       bout, bclr, io);
-  else
-    hipLaunchKernelGGL((k_step_tiled<LANES, false, OCC>), dim3(n_tiles), dim3(block), lds, st, mesh, reps, tiles,
+    hipLaunchKernelGGL((k_step_tiled<LANES, true, OCC, false>), dim3(n_tiles), dim3(block), lds, st, mesh, reps, tiles,
                        bin, bout, bclr, io);
+  else if (io.visit)  // the visit check: the same step, recording each particle's visited set
+    hipLaunchKernelGGL((k_step_tiled<LANES, false, OCC, true>), dim3(n_tiles), dim3(block), lds, st, mesh, reps, tiles,
+                       bin, bout, bclr, io);
+  else
+    hipLaunchKernelGGL((k_step_tiled<LANES, false, OCC, false>), dim3(n_tiles), dim3(block), lds, st, mesh, reps,
+                       tiles, bin, bout, bclr, io);
 }
 
 template <int LANES>

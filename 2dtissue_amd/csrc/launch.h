@@ -83,6 +83,8 @@ struct TiledIO {
   int step;                 // step index within the run
   float4* tape_fs;          // adjoint tape of this step, [ntot] (Fx, Fy, Sx, Sy); used by the TAPE variant
   int* tape_q;              // [ntot] quarter turns of the seam wrap
+  int2* visit;              // [ntot] (na, vh) of StepOut: the neighbour set the step's own walk
+                            //   visited (-1, 0 for a replica passed through); null: not recorded
 };
 
 struct ResidentIO {

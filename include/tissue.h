@@ -115,6 +115,15 @@ int32_t tissue_get_flags(tissue_sim* s, uint32_t* flags, int32_t* first_step);
 /* neighbour sets {j : d_ij < 2 sigma} (self included) of the current state, at most cap per
  * particle: counts[n], cols[n*cap], each list ascending */
 int32_t tissue_neighbor_lists(tissue_sim* s, int32_t* counts, int32_t* cols, int32_t cap);
+/* Visit check (no counterpart in the reference; a diagnostic for parity tests of the step's own
+ * neighbour walk, which the reference performs as the dense loops of Locomotion.cpp:94-111 and
+ * ForceHelper.cpp:34-73). While enabled, every step of the step kernel (TISSUE_MODE_STEP_KERNEL)
+ * records per particle the number of particles its walk visited within 2 sigma (self included) and
+ * the sum, modulo 2^32, of h(j) over their indices j (counted over all replicas' particles), with
+ * h the MurmurHash3 32-bit finaliser; tissue_get_visits returns those of the last step (-1 for a
+ * particle not stepped since the check was enabled or whose replica was stopped). */
+int32_t tissue_set_visit_check(tissue_sim* s, int32_t enable);
+int32_t tissue_get_visits(tissue_sim* s, int32_t* na, uint32_t* hash);
 /* launch shape: lanes per particle (1, 2, 4, 8, 16, 32, 64) and threads per block (64..512); 0 = auto */
 int32_t tissue_set_launch(tissue_sim* s, int32_t lanes, int32_t block);
 /* execution mode of tissue_step*: one kernel launch per step over the whole GPU, or the

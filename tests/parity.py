@@ -92,10 +92,32 @@ def compare_phi(s_out, out, ref_s, ref_o, phi_gpu):
     return fails, int(exempt.sum())
 
 
-def compare_step(O, om, p, s_in, s_out, out, nbrs_gpu, eta, sparse=False, phi_gpu=None):
+def visit_mix(j):
+    """The MurmurHash3 32-bit finaliser, elementwise on uint32 (the step kernel's visit_mix)."""
+    h = np.asarray(j, dtype=np.uint32).copy()
+    h ^= h >> np.uint32(16)
+    h *= np.uint32(0x85EBCA6B)
+    h ^= h >> np.uint32(13)
+    h *= np.uint32(0xC2B2AE35)
+    h ^= h >> np.uint32(16)
+    return h
+
+
+def visit_records(sets, off=0):
+    """(na, hash) the visit check must report for neighbour sets given as ids within a replica
+    starting at global index off: the set sizes and the sums mod 2^32 of visit_mix(off + j)."""
+    na = np.array([len(s) for s in sets], np.int32)
+    h = np.array([int(visit_mix(np.asarray(s, np.int64) + off).sum(dtype=np.uint64) & 0xFFFFFFFF) for s in sets],
+                 np.uint32)
+    return na, h
+
+
+def compare_step(O, om, p, s_in, s_out, out, nbrs_gpu, eta, sparse=False, phi_gpu=None, visits=None):
     """Compare one GPU step (s_in -> s_out, outputs) with the oracle's step from the same inputs.
     sparse selects the oracle's O(N) step, bitwise equal to the dense one. phi_gpu: the step's
-    (phi_ref, phi_polar) from the GPU (compared when given). Returns a dict of failure counts (all
+    (phi_ref, phi_polar) from the GPU (compared when given). visits: the step's own (na, hash)
+    records (Sim.visits), compared exactly with the oracle's neighbour sets of the input state when
+    given. Returns a dict of failure counts (all
     zero when the step matches); the count of particles exempt from the phi_ref sign comparison is
     returned in the second value's "phi_exempt"."""
     if sparse:
@@ -112,6 +134,9 @@ def compare_step(O, om, p, s_in, s_out, out, nbrs_gpu, eta, sparse=False, phi_gp
     fails["nbr"] = int((out.nbr != ref_o["nbr"]).sum())
     ref_sets = (O.neighbor_sets_sparse if sparse else O.neighbor_sets)(om, s_in.vid, p["sigma"])
     fails["sets"] = int(sum(not np.array_equal(a, b) for a, b in zip(nbrs_gpu, ref_sets)))
+    if visits is not None:
+        na, h = visit_records(ref_sets)
+        fails["visits"] = int(((visits[0] != na) | (visits[1] != h)).sum())
     r3d, vid, face = O.remap(om, s_out.x, s_out.y, 1)
     fails["vid"] = int((vid != s_out.vid).sum())
     fails["face"] = int((face != out.face).sum())

@@ -21,6 +21,8 @@ def test_large_surface_teacher_forced(T, O, first):
     O.set_threads(16)
     p = dict(P.PHYS, sigma=cfg["sigma"])
     sim = T.Sim(m, T.params(**p), cfg["n"])
+    sim.set_mode(T.MODE_STEP_KERNEL)
+    sim.set_visit_check(True)
     sim.init_random(cfg["seed"])
     noise = T.gen_noise(cfg["noise_seed"], 0.1, first + 2, cfg["n"])
     sim.stage_noise(noise)
@@ -31,7 +33,8 @@ def test_large_surface_teacher_forced(T, O, first):
         nb = sim.neighbor_lists()
         assert sim.step(1, noise[t]) == T.OK
         fails, ref = P.compare_step(O, om, p, s_in, sim.get_state(), sim.get_outputs(), nb,
-                                    noise[t].astype(np.float64), sparse=True, phi_gpu=P.last_phi(sim))
+                                    noise[t].astype(np.float64), sparse=True, phi_gpu=P.last_phi(sim),
+                                    visits=sim.visits())
         print(t, fails, f"phi_ref sign exempt: {ref['phi_exempt']}", flush=True)
         for k, v in fails.items():
             total[k] = total.get(k, 0) + v

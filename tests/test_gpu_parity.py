@@ -9,13 +9,18 @@ import parity as P
 pytestmark = pytest.mark.gpu
 
 
-def run_teacher_forced(T, O, cfg, steps, lanes=0, mode=0, sparse=False):
+def run_teacher_forced(T, O, cfg, steps, lanes=0, mode=0, sparse=False, block=0):
+    """Teacher-forced run; with the step kernel (mode 1) the step's own neighbour walk is checked
+    too: each particle's visited set (size and order-free id hash) against the oracle's set."""
     m, tb = P.mesh_and_tables(cfg["freq"], cfg["sigma"])
     om = O.Mesh(tb)
     p = dict(P.PHYS, sigma=cfg["sigma"])
     sim = T.Sim(m, T.params(**p), cfg["n"])
     sim.set_mode(mode)
-    sim.set_launch(lanes, 0)
+    sim.set_launch(lanes, block)
+    check_visits = mode == 1
+    if check_visits:
+        sim.set_visit_check(True)
     sim.init_random(cfg["seed"])
     noise = T.gen_noise(cfg["noise_seed"], 0.1, steps, cfg["n"])
     total = {}
@@ -27,7 +32,7 @@ def run_teacher_forced(T, O, cfg, steps, lanes=0, mode=0, sparse=False):
         s_out = sim.get_state()
         out = sim.get_outputs()
         fails, ref = P.compare_step(O, om, p, s_in, s_out, out, nb, noise[t].astype(np.float64), sparse,
-                                    phi_gpu=P.last_phi(sim))
+                                    phi_gpu=P.last_phi(sim), visits=sim.visits() if check_visits else None)
         exempt += ref["phi_exempt"]
         for k, v in fails.items():
             total[k] = total.get(k, 0) + v
@@ -49,6 +54,18 @@ def test_c1_teacher_forced_50_steps(T, O, lanes, mode):
 @pytest.mark.parametrize("mode", [1, 2])
 def test_c2_teacher_forced_20_steps(T, O, mode):
     total = run_teacher_forced(T, O, P.C2, 20, 0, mode)
+    assert all(v == 0 for v in total.values()), total
+
+
+@pytest.mark.parametrize("cfg,lanes,block", [("C1", 2, 128), ("C2", 2, 128), ("C2", 4, 256), ("C2", 8, 512),
+                                              ("C1", 16, 512)])
+def test_occupancy_mask_walk_teacher_forced(T, O, cfg, lanes, block):
+    """Launch shapes whose workgroups stage the vertex-occupancy mask in LDS (the batched shape,
+    2 lanes in 128 threads; 4 lanes in 256; and at 8 and 16 lanes the walk that loads bucket
+    heads in its scan): the walk that skips unoccupied entries from LDS visits exactly the
+    oracle's neighbour sets (visit check) and the step matches the oracle."""
+    total = run_teacher_forced(T, O, getattr(P, cfg), 20, lanes, 1, block=block)
+    assert total.get("visits", -1) == 0
     assert all(v == 0 for v in total.values()), total
 
 

@@ -427,3 +427,23 @@ def test_trajectory_divergence_horizon(O):
     assert far[0] == 0          # one step: FP32 storage alone stays far below 1e-5
     assert far[2] > 10          # three steps: some particles have parted
     assert far[7] > 0.9 * n     # eight steps: nearly all have
+
+
+def test_visit_records_name_the_set():
+    """The visit check's records (set size, sum of the MurmurHash3 finaliser over the ids) change
+    when a set loses, gains or swaps a member, and not when it is reordered. The finaliser is
+    checked against its known answers (fmix32(0) = 0, fmix32(1) = 0x514E28B7)."""
+    import parity as P
+    assert int(P.visit_mix(0)) == 0 and int(P.visit_mix(1)) == 0x514E28B7
+    rng = np.random.default_rng(9)
+    sets = [np.sort(rng.choice(5000, size=k, replace=False)) for k in rng.integers(1, 12, 200)]
+    na, h = P.visit_records(sets, off=300)
+    na2, h2 = P.visit_records([s[::-1] for s in sets], off=300)
+    assert np.array_equal(na, na2) and np.array_equal(h, h2)
+    for i in range(0, 200, 7):
+        s = sets[i]
+        dropped = P.visit_records([s[1:]], off=300)
+        added = P.visit_records([np.append(s, 4999 if 4999 not in s else 4998)], off=300)
+        swapped = P.visit_records([np.where(s == s[0], (s[0] + 1) % 5000 if s[0] + 1 not in s else 4997, s)], off=300)
+        assert dropped[0][0] != na[i] and added[0][0] != na[i]
+        assert swapped[1][0] != h[i] and swapped[0][0] == na[i]
