@@ -67,13 +67,28 @@ __global__ void __launch_bounds__(kStepMaxBlock) k_step_tiled(MeshDev mesh, cons
   const int r = tile.r;
   const ReplicaDesc& rep = reps[r];  // uniform: read through the scalar cache where used
   const int nv = mesh.nv, tid = threadIdx.x, nw = occ_words(nv);
+  const int lane = tid & (LANES - 1);
+  const int i = tile.first + tid / LANES;
+  const bool active = i < rep.n;
+  const int g = rep.off + i;
+  // the particle's inputs first: their loads are in flight during the set-up below
+  float4 me = make_float4(0.f, 0.f, 0.f, 0.f);
+  float th = 0.f, eta = 0.f;
+  int v = 0;
+  if (active) {
+    me = io.xycs_in[g];
+    th = io.th_in[g];
+    v = io.vid_in[g];
+    const size_t shift = io.noise_row0 ? (size_t)io.noise_row0[0] * io.noise_stride : 0;
+    eta = io.noise ? io.noise[shift + g] : 0.0f;
+  }
   if constexpr (OCC) {  // stage this replica's occupancy mask of the input buckets in LDS
     const unsigned* occ_in = bin.occ + (size_t)r * nw;
     for (int k = tid; k < nw; k += (int)blockDim.x) s_occ[k] = occ_in[k];
   }
   // clear this replica's bucket set consumed two steps ago (it is next step's output)
   int* clr = bclr.count + (size_t)r * nv;
-  for (int v = tile.q * (int)blockDim.x + tid; v < nv; v += tile.nq * (int)blockDim.x) clr[v] = 0;
+  for (int w = tile.q * (int)blockDim.x + tid; w < nv; w += tile.nq * (int)blockDim.x) clr[w] = 0;
   unsigned* clr_occ = bclr.occ + (size_t)r * nw;
   for (int k = tile.q * (int)blockDim.x + tid; k < nw; k += tile.nq * (int)blockDim.x) clr_occ[k] = 0u;
   __syncthreads();
@@ -88,16 +103,10 @@ __global__ void __launch_bounds__(kStepMaxBlock) k_step_tiled(MeshDev mesh, cons
     if (halt) atomicOr(&io.halt_out[r], halt);
   }
   const BucketsOut bo = replica_out(bout, r, nv, rep.off);
-  const int lane = tid & (LANES - 1);
-  const int i = tile.first + tid / LANES;
 
   int sgn = 0;
   long long pcs = 0, psn = 0;
-  if (i < rep.n) {
-    const int g = rep.off + i;
-    const float4 me = io.xycs_in[g];
-    const float th = io.th_in[g];
-    const int v = io.vid_in[g];
+  if (active) {
     if (halt) {
       if (lane == 0) {
         io.xycs_out[g] = me;
@@ -107,8 +116,6 @@ __global__ void __launch_bounds__(kStepMaxBlock) k_step_tiled(MeshDev mesh, cons
         if constexpr (VISITS) io.visit[g] = make_int2(-1, 0);
       }
     } else {
-      const size_t shift = io.noise_row0 ? (size_t)io.noise_row0[0] * io.noise_stride : 0;
-      const float eta = io.noise ? io.noise[shift + g] : 0.0f;
       const GlobalSource<OCC> src{replica_in(bin, r, nv, rep.off), io.xycs_in, io.th_in, s_occ};
       const StepOut o = particle_step<LANES>(mesh, rep.prm, src, g, me, th, v, eta, lane);
       if (lane == 0) {
