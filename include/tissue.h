@@ -120,8 +120,9 @@ int32_t tissue_neighbor_lists(tissue_sim* s, int32_t* counts, int32_t* cols, int
  * ForceHelper.cpp:34-73). While enabled, every step of the step kernel (TISSUE_MODE_STEP_KERNEL)
  * records per particle the number of particles its walk visited within 2 sigma (self included) and
  * the sum, modulo 2^32, of h(j) over their indices j (counted over all replicas' particles), with
- * h the MurmurHash3 32-bit finaliser; tissue_get_visits returns those of the last step (-1 for a
- * particle not stepped since the check was enabled or whose replica was stopped). */
+ * h the MurmurHash3 32-bit finaliser; tissue_get_visits returns those of the last step of the step
+ * kernel (-1 for a particle it has not stepped since the check was enabled, or whose replica was
+ * stopped; steps of the persistent kernel, TISSUE_MODE_RESIDENT, record nothing). */
 int32_t tissue_set_visit_check(tissue_sim* s, int32_t enable);
 int32_t tissue_get_visits(tissue_sim* s, int32_t* na, uint32_t* hash);
 /* launch shape: lanes per particle (1, 2, 4, 8, 16, 32, 64) and threads per block (64..512); 0 = auto */
