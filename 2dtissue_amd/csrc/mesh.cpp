@@ -633,8 +633,41 @@ void build_face_buckets(UVMesh& m, int grid) {
       c0[c] = std::clamp((int)std::floor((lo[c] - eps) * grid), 0, grid - 1);
       c1[c] = std::clamp((int)std::floor((hi[c] + eps) * grid), 0, grid - 1);
     }
+    // of the cells under the face's bounding box, only those the triangle meets (each cell grown
+    // by eps): the box's own axes are settled by the bounding box, so a cell is skipped when one of
+    // the triangle's edge normals separates them (about a quarter of the bounding-box cells)
+    const double* P[3] = {&m.uv[2 * m.faces[3 * f]], &m.uv[2 * m.faces[3 * f + 1]], &m.uv[2 * m.faces[3 * f + 2]]};
+    double nx[3], ny[3], tlo[3], thi[3];
+    for (int e = 0; e < 3; ++e) {
+      const double* a = P[e];
+      const double* b = P[(e + 1) % 3];
+      nx[e] = -(b[1] - a[1]);
+      ny[e] = b[0] - a[0];
+      tlo[e] = 1e300;
+      thi[e] = -1e300;
+      for (int q = 0; q < 3; ++q) {
+        const double t = nx[e] * P[q][0] + ny[e] * P[q][1];
+        tlo[e] = std::min(tlo[e], t);
+        thi[e] = std::max(thi[e], t);
+      }
+    }
     for (int cy = c0[1]; cy <= c1[1]; ++cy)
-      for (int cx = c0[0]; cx <= c1[0]; ++cx) cells[(size_t)cy * grid + cx].push_back(f);
+      for (int cx = c0[0]; cx <= c1[0]; ++cx) {
+        const double bx[2] = {(double)cx / grid - eps, (double)(cx + 1) / grid + eps};
+        const double by[2] = {(double)cy / grid - eps, (double)(cy + 1) / grid + eps};
+        bool separated = false;
+        for (int e = 0; e < 3 && !separated; ++e) {
+          double blo = 1e300, bhi = -1e300;
+          for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j) {
+              const double t = nx[e] * bx[i] + ny[e] * by[j];
+              blo = std::min(blo, t);
+              bhi = std::max(bhi, t);
+            }
+          separated = bhi < tlo[e] || thi[e] < blo;
+        }
+        if (!separated) cells[(size_t)cy * grid + cx].push_back(f);
+      }
   }
   m.cell_start.assign((size_t)grid * grid + 1, 0);
   m.cell_faces.clear();
