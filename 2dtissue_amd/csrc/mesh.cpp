@@ -616,7 +616,10 @@ UVMesh cut_and_parameterize(const SurfaceMesh& m, int pa, int pm, int pb) {
 // ------------------------------------------------------------------ face buckets
 
 void build_face_buckets(UVMesh& m, int grid) {
-  if (grid <= 0) grid = std::max(1, (int)std::ceil(std::sqrt(1.0 * m.nf)));
+  // automatic: 1.5 sqrt(F) cells a side, about 3.6 candidate faces per cell (measured on MI355X at
+  // 20,480 faces: 1.0 sqrt(F) cells, 5.2 faces, batched step 186.6 us; 1.5 sqrt(F), 182.0 us; 2.0,
+  // 180.1 us with the single-system step 0.2 % slower; the 100,000-particle surface unchanged)
+  if (grid <= 0) grid = std::max(1, (int)std::ceil(1.5 * std::sqrt(1.0 * m.nf)));
   m.grid = grid;
   const double eps = 1e-9;
   std::vector<std::vector<int32_t>> cells((size_t)grid * grid);

@@ -33,7 +33,7 @@ struct UVMesh {
   std::vector<int32_t> orig;      // nv: UV vertex -> closed-surface vertex
   std::vector<int32_t> cut_path;  // closed-surface ids Pa ... Pm ... Pb
   int32_t pa = -1, pm = -1, pb = -1;
-  // face buckets: grid x grid cells over [0,1]^2, CSR lists of faces whose UV bbox meets the cell
+  // face buckets: grid x grid cells over [0,1]^2, CSR lists of the faces whose UV triangle meets the cell (grown by 1e-9)
   int32_t grid = 0;
   std::vector<int32_t> cell_start, cell_faces;
   // truncated geodesic table on the open mesh: CSR rows sorted by (dist, col); dist <= cutoff;
