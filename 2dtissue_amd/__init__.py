@@ -116,6 +116,8 @@ SIGNATURES = [
     ("tissue_mesh_load_off", C.c_int32, [C.c_char_p, C.POINTER(_P)]),
     ("tissue_mesh_build_tables", C.c_int32, [_P, C.c_double, C.c_int32]),
     ("tissue_mesh_build_tables_on", C.c_int32, [_P, C.c_double, C.c_int32, C.c_int32]),
+    ("tissue_set_uv_solve", C.c_int32, [C.c_int32]),
+    ("tissue_get_uv_solve", C.c_int32, [_I]),
     ("tissue_mesh_destroy", None, [_P]),
     ("tissue_mesh_info", C.c_int32, [_P, _I, _I, _I, C.POINTER(C.c_int64), _I, C.POINTER(C.c_int64)]),
     ("tissue_mesh_export", C.c_int32, [_P, _D, _D, _I, _I, _I, _F, _I, _I]),
@@ -198,6 +200,18 @@ def _ptr(a: np.ndarray | None, ct):
         return None
     assert a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(C.POINTER(ct))
+
+
+def set_uv_solve(where: int) -> None:
+    """Where the mesh constructors solve the UV map (tissue_set_uv_solve): BUILD_AUTO, BUILD_HOST or
+    BUILD_GPU; the result is the same bit for bit."""
+    _check(lib().tissue_set_uv_solve(where))
+
+
+def get_uv_solve() -> int:
+    w = C.c_int32()
+    _check(lib().tissue_get_uv_solve(C.byref(w)))
+    return w.value
 
 
 class Mesh:

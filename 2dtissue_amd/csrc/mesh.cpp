@@ -305,7 +305,7 @@ std::vector<std::unordered_map<int, double>> mean_value_weights(const std::vecto
 
 }  // namespace
 
-UVMesh cut_and_parameterize(const SurfaceMesh& m, int pa, int pm, int pb) {
+UVMesh cut_and_parameterize(const SurfaceMesh& m, int pa, int pm, int pb, const UvSolver& solver) {
   const int V = m.nv(), F = m.nf();
   if (V < 4 || F < 4) throw std::runtime_error("cut: mesh too small");
   // Euler characteristic check (closed genus 0: V - E + F = 2)
@@ -450,13 +450,13 @@ UVMesh cut_and_parameterize(const SurfaceMesh& m, int pa, int pm, int pb) {
     rp[r + 1] = (int)ci.size();
   }
   // Jacobi-preconditioned CG for both coordinates at once (the two systems share the matrix),
-  // rows split into fixed blocks of kBlock: every dot product is the sum of per-block partials
+  // rows split into fixed blocks of kUvBlock: every dot product is the sum of per-block partials
   // (rows in order within a block) added in block order, so the solution does not depend on the
   // number of threads. Threads take blocks round-robin and meet at a barrier after each of the
   // three phases of an iteration; every thread forms the same scalars from the partials and so
   // takes the same convergence decisions. A coordinate stops updating once converged (each
   // coordinate's arithmetic is that of its own solve).
-  constexpr int kBlock = 2048;
+  constexpr int kBlock = kUvBlock;
   const int nb = (ni + kBlock - 1) / kBlock;
   const std::vector<double>* bs[2] = {&bx, &by};
   std::vector<double> sol[2], res[2], zz[2], pp[2], Ap[2];
@@ -483,8 +483,26 @@ UVMesh cut_and_parameterize(const SurfaceMesh& m, int pa, int pm, int pb) {
     live[k] = bn[k] != 0;
   }
   bool converged = !live[0] && !live[1];
+  const int max_it = 20 * ni + 100;
+  if (!converged && solver) {  // the same iteration elsewhere (the GPU), if it takes the system
+    UvSystem sys;
+    sys.ni = ni;
+    sys.nb = nb;
+    sys.max_it = max_it;
+    sys.rp = rp.data();
+    sys.ci = ci.data();
+    sys.val = val.data();
+    sys.diag = diag.data();
+    for (int k = 0; k < 2; ++k) {
+      sys.b[k] = bs[k]->data();
+      sys.z0[k] = zz[k].data();
+      sys.rz[k] = rz[k];
+      sys.bn[k] = bn[k];
+      sys.live[k] = live[k];
+    }
+    converged = solver(sys, sol);
+  }
   if (!converged) {
-    const int max_it = 20 * ni + 100;
     const int threads = std::min(build_threads(), nb);
     run_team(threads, [&](int me, int nt, SpinBarrier& bar) {
       double rzl[2] = {rz[0], rz[1]};
@@ -557,7 +575,7 @@ UVMesh cut_and_parameterize(const SurfaceMesh& m, int pa, int pm, int pb) {
         double beta[2] = {0, 0};
         for (int k = 0; k < 2; ++k) {
           if (!on[k]) continue;
-          if (std::sqrt(total(2 + k)) <= 1e-13 * bn[k]) {
+          if (std::sqrt(total(2 + k)) <= kUvTol * bn[k]) {
             on[k] = false;
             continue;
           }
@@ -623,55 +641,64 @@ void build_face_buckets(UVMesh& m, int grid) {
   m.grid = grid;
   const double eps = 1e-9;
   std::vector<std::vector<int32_t>> cells((size_t)grid * grid);
-  for (int f = 0; f < m.nf; ++f) {
-    double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300};
-    for (int q = 0; q < 3; ++q)
-      for (int c = 0; c < 2; ++c) {
-        double x = m.uv[2 * m.faces[3 * f + q] + c];
-        lo[c] = std::min(lo[c], x);
-        hi[c] = std::max(hi[c], x);
-      }
-    int c0[2], c1[2];
-    for (int c = 0; c < 2; ++c) {
-      c0[c] = std::clamp((int)std::floor((lo[c] - eps) * grid), 0, grid - 1);
-      c1[c] = std::clamp((int)std::floor((hi[c] + eps) * grid), 0, grid - 1);
-    }
-    // of the cells under the face's bounding box, only those the triangle meets (each cell grown
-    // by eps): the box's own axes are settled by the bounding box, so a cell is skipped when one of
-    // the triangle's edge normals separates them (about a quarter of the bounding-box cells)
-    const double* P[3] = {&m.uv[2 * m.faces[3 * f]], &m.uv[2 * m.faces[3 * f + 1]], &m.uv[2 * m.faces[3 * f + 2]]};
-    double nx[3], ny[3], tlo[3], thi[3];
-    for (int e = 0; e < 3; ++e) {
-      const double* a = P[e];
-      const double* b = P[(e + 1) % 3];
-      nx[e] = -(b[1] - a[1]);
-      ny[e] = b[0] - a[0];
-      tlo[e] = 1e300;
-      thi[e] = -1e300;
-      for (int q = 0; q < 3; ++q) {
-        const double t = nx[e] * P[q][0] + ny[e] * P[q][1];
-        tlo[e] = std::min(tlo[e], t);
-        thi[e] = std::max(thi[e], t);
-      }
-    }
-    for (int cy = c0[1]; cy <= c1[1]; ++cy)
-      for (int cx = c0[0]; cx <= c1[0]; ++cx) {
-        const double bx[2] = {(double)cx / grid - eps, (double)(cx + 1) / grid + eps};
-        const double by[2] = {(double)cy / grid - eps, (double)(cy + 1) / grid + eps};
-        bool separated = false;
-        for (int e = 0; e < 3 && !separated; ++e) {
-          double blo = 1e300, bhi = -1e300;
-          for (int i = 0; i < 2; ++i)
-            for (int j = 0; j < 2; ++j) {
-              const double t = nx[e] * bx[i] + ny[e] * by[j];
-              blo = std::min(blo, t);
-              bhi = std::max(bhi, t);
-            }
-          separated = bhi < tlo[e] || thi[e] < blo;
+  // threads take bands of cell rows; each walks every face in order and fills only its band's
+  // cells, so a cell lists its faces in ascending order whatever the number of threads
+  const int threads = std::max(1, std::min(build_threads(), grid));
+  run_team(threads, [&](int me, int nt, SpinBarrier&) {
+    const int band0 = (int)((long)grid * me / nt), band1 = (int)((long)grid * (me + 1) / nt);
+    for (int f = 0; f < m.nf; ++f) {
+      double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300};
+      for (int q = 0; q < 3; ++q)
+        for (int c = 0; c < 2; ++c) {
+          double x = m.uv[2 * m.faces[3 * f + q] + c];
+          lo[c] = std::min(lo[c], x);
+          hi[c] = std::max(hi[c], x);
         }
-        if (!separated) cells[(size_t)cy * grid + cx].push_back(f);
+      int c0[2], c1[2];
+      for (int c = 0; c < 2; ++c) {
+        c0[c] = std::clamp((int)std::floor((lo[c] - eps) * grid), 0, grid - 1);
+        c1[c] = std::clamp((int)std::floor((hi[c] + eps) * grid), 0, grid - 1);
       }
-  }
+      c0[1] = std::max(c0[1], band0);
+      c1[1] = std::min(c1[1], band1 - 1);
+      if (c0[1] > c1[1]) continue;
+      // of the cells under the face's bounding box, only those the triangle meets (each cell grown
+      // by eps): the box's own axes are settled by the bounding box, so a cell is skipped when one of
+      // the triangle's edge normals separates them (about a quarter of the bounding-box cells)
+      const double* P[3] = {&m.uv[2 * m.faces[3 * f]], &m.uv[2 * m.faces[3 * f + 1]], &m.uv[2 * m.faces[3 * f + 2]]};
+      double nx[3], ny[3], tlo[3], thi[3];
+      for (int e = 0; e < 3; ++e) {
+        const double* a = P[e];
+        const double* b = P[(e + 1) % 3];
+        nx[e] = -(b[1] - a[1]);
+        ny[e] = b[0] - a[0];
+        tlo[e] = 1e300;
+        thi[e] = -1e300;
+        for (int q = 0; q < 3; ++q) {
+          const double t = nx[e] * P[q][0] + ny[e] * P[q][1];
+          tlo[e] = std::min(tlo[e], t);
+          thi[e] = std::max(thi[e], t);
+        }
+      }
+      for (int cy = c0[1]; cy <= c1[1]; ++cy)
+        for (int cx = c0[0]; cx <= c1[0]; ++cx) {
+          const double bx[2] = {(double)cx / grid - eps, (double)(cx + 1) / grid + eps};
+          const double by[2] = {(double)cy / grid - eps, (double)(cy + 1) / grid + eps};
+          bool separated = false;
+          for (int e = 0; e < 3 && !separated; ++e) {
+            double blo = 1e300, bhi = -1e300;
+            for (int i = 0; i < 2; ++i)
+              for (int j = 0; j < 2; ++j) {
+                const double t = nx[e] * bx[i] + ny[e] * by[j];
+                blo = std::min(blo, t);
+                bhi = std::max(bhi, t);
+              }
+            separated = bhi < tlo[e] || thi[e] < blo;
+          }
+          if (!separated) cells[(size_t)cy * grid + cx].push_back(f);
+        }
+    }
+  });
   m.cell_start.assign((size_t)grid * grid + 1, 0);
   m.cell_faces.clear();
   for (size_t c = 0; c < cells.size(); ++c) {

@@ -10,6 +10,7 @@
 // copy and an FP64 host copy hold identical values.
 #pragma once
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <utility>
 #include <vector>
@@ -52,8 +53,31 @@ bool save_off(const std::string& path, const double* xyz, int nv, const int32_t*
               std::string& err);
 
 // ---- cut + UV map (throws std::runtime_error on failure) ----
+// The interior of the UV map solves A x = b for both coordinates (A: the symmetric mean-value
+// Laplacian of the free vertices, CSR off-diagonals rp/ci/val plus diag) by Jacobi-preconditioned
+// CG with fixed reduction blocks: every dot product is the sum, in block order, of per-block sums
+// of kUvBlock consecutive rows taken in row order, so the arithmetic is the same on any number of
+// threads and on the GPU (uv_solve_gpu). A coordinate stops at |r| <= kUvTol |b|.
+constexpr int kUvBlock = 2048;
+constexpr double kUvTol = 1e-13;
+struct UvSystem {
+  int ni = 0, nb = 0, max_it = 0;  // rows, reduction blocks, iteration limit
+  const int* rp = nullptr;
+  const int* ci = nullptr;
+  const double* val = nullptr;
+  const double* diag = nullptr;
+  const double* b[2] = {nullptr, nullptr};   // right-hand sides: the starting residuals
+  const double* z0[2] = {nullptr, nullptr};  // starting preconditioned residuals = directions
+  double rz[2] = {0, 0}, bn[2] = {0, 0};     // r.z and |b| at the start
+  bool live[2] = {false, false};             // coordinates with a nonzero right-hand side
+};
+// An alternative solver of the same iteration: returns true with both solutions in sol[0..1], or
+// false to leave the system to the host; throws when the iteration does not converge.
+using UvSolver = std::function<bool(const UvSystem&, std::vector<double>* sol)>;
+
 // pa/pm/pb: closed-surface vertex ids of the cone points; pass -1 to choose automatically.
-UVMesh cut_and_parameterize(const SurfaceMesh& m, int pa = -1, int pm = -1, int pb = -1);
+UVMesh cut_and_parameterize(const SurfaceMesh& m, int pa = -1, int pm = -1, int pb = -1,
+                            const UvSolver& solver = nullptr);
 
 // ---- acceleration structures ----
 void build_face_buckets(UVMesh& uv, int grid = 0);        // grid <= 0: automatic

@@ -78,6 +78,16 @@ enum { TISSUE_BUILD_AUTO = -1, TISSUE_BUILD_HOST = 0, TISSUE_BUILD_GPU = 1 };
  * the host's threads; TISSUE_BUILD_GPU is rejected (TISSUE_E_INVALID_ARG): the table has no GPU
  * build. Replaces the same reference call as tissue_mesh_build_tables. */
 int32_t tissue_mesh_build_tables_on(tissue_mesh* m, double sigma_max, int32_t grid, int32_t where);
+/* Where the mesh constructors (tissue_mesh_ellipsoid, tissue_mesh_perturbed_sphere,
+ * tissue_mesh_load_off) solve the UV map's harmonic system, process-wide: TISSUE_BUILD_AUTO (the
+ * default: the current GPU when one is visible and the system has 100,000 free vertices or more,
+ * else the host's threads), TISSUE_BUILD_HOST, or TISSUE_BUILD_GPU (a constructor then fails with
+ * TISSUE_E_HIP when no GPU is visible). The GPU runs the host's
+ * iteration operation for operation, so the UV map is the same bit for bit either way. Part of
+ * the UV chart that replaces MeshCartographyLib's create_uv_surface (2DTissue.cpp:85-91), whose
+ * source is absent. */
+int32_t tissue_set_uv_solve(int32_t where);
+int32_t tissue_get_uv_solve(int32_t* where);
 void tissue_mesh_destroy(tissue_mesh* m);
 int32_t tissue_mesh_info(const tissue_mesh* m, int32_t* nv, int32_t* nf, int32_t* nv3d, int64_t* nnz, int32_t* grid,
                          int64_t* n_cell_faces);

@@ -121,8 +121,9 @@ def test_distance_table_structure(ell):
 
 
 def test_mesh_build_independent_of_thread_count(tmp_path):
-    """The UV solve and the geodesic table are split over host threads; their results must not
-    depend on how many: the same mesh built with 1 and with 5 threads, byte for byte."""
+    """The UV solve, the face buckets and the geodesic table are split over host threads; their
+    results must not depend on how many: the same mesh built with 1 and with 5 threads, byte for
+    byte."""
     import subprocess
     import sys
     code = (
@@ -247,3 +248,18 @@ def test_random_init_host_is_area_uniform(T, O, ell):
     share = area[upper].sum() / area.sum()
     got = upper[face].mean()
     assert abs(got - share) < 4 * np.sqrt(share * (1 - share) / n), (got, share)
+
+
+def test_uv_solve_place_is_validated(T):
+    """tissue_set_uv_solve takes AUTO, HOST or GPU; anything else is rejected and leaves it as it
+    was; HOST builds on the host in any case."""
+    assert T.get_uv_solve() == T.BUILD_AUTO
+    with pytest.raises(T.TissueError):
+        T.set_uv_solve(7)
+    assert T.get_uv_solve() == T.BUILD_AUTO
+    T.set_uv_solve(T.BUILD_HOST)
+    try:
+        m = T.Mesh.ellipsoid(2.0, 1.0, 1.0, 8)
+        assert m.info()["nf"] == 20 * 8 * 8
+    finally:
+        T.set_uv_solve(T.BUILD_AUTO)
