@@ -161,6 +161,8 @@ SIGNATURES = [
     ("tissue_batch_get_launch", C.c_int32, [_P, _I]),
     ("tissue_batch_autotune", C.c_int32, [_P, C.c_int32, _I, _I]),
     ("tissue_batch_last_run_ms", C.c_int32, [_P, _D]),
+    ("tissue_batch_set_visit_check", C.c_int32, [_P, C.c_int32]),
+    ("tissue_batch_get_visits", C.c_int32, [_P, _I, _I]),
     ("tissue_adjoint", C.c_int32, [_P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(Loss), C.POINTER(Grads), _D, _D,
                                    _D]),
     ("tissue_batch_adjoint", C.c_int32, [_P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(Loss), C.POINTER(Grads), _D,
@@ -576,6 +578,16 @@ class Batch:
         ms = C.c_double()
         _check(lib().tissue_batch_last_run_ms(self.h, C.byref(ms)))
         return ms.value
+
+    def set_visit_check(self, enable: bool = True) -> None:
+        """Sim.set_visit_check over every replica (tissue_batch_set_visit_check)."""
+        _check(lib().tissue_batch_set_visit_check(self.h, 1 if enable else 0))
+
+    def visits(self) -> tuple[np.ndarray, np.ndarray]:
+        """(na, hash) over all particles for the last step (indices hashed globally)."""
+        na = np.zeros(self.ntot, np.int32); h = np.zeros(self.ntot, np.uint32)
+        _check(lib().tissue_batch_get_visits(self.h, _ptr(na, C.c_int32), _ptr(h, C.c_int32)))
+        return na, h
 
     def close(self):
         if getattr(self, "h", None):

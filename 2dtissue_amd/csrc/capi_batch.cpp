@@ -161,6 +161,20 @@ int32_t tissue_batch_autotune(tissue_batch* b, int32_t n_steps, int32_t* lanes_o
   });
 }
 
+int32_t tissue_batch_set_visit_check(tissue_batch* b, int32_t enable) {
+  return guarded([&] {
+    eng(b).set_visit_check(enable != 0);
+    return TISSUE_OK;
+  });
+}
+
+int32_t tissue_batch_get_visits(tissue_batch* b, int32_t* na, uint32_t* hash) {
+  return guarded([&] {
+    eng(b).get_visits(na, hash);
+    return TISSUE_OK;
+  });
+}
+
 int32_t tissue_batch_last_run_ms(tissue_batch* b, double* ms) {
   return guarded([&] {
     if (!ms) throw ApiError(TISSUE_E_INVALID_ARG, "null argument");

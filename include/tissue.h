@@ -194,6 +194,10 @@ int32_t tissue_batch_get_launch(tissue_batch* b, int32_t* shape);
 int32_t tissue_batch_autotune(tissue_batch* b, int32_t n_steps, int32_t* lanes, int32_t* block);
 /* GPU time of the last tissue_batch_run (HIP events around the graph launch) */
 int32_t tissue_batch_last_run_ms(tissue_batch* b, double* ms);
+/* the visit check of tissue_set_visit_check over all replicas' particles ([total particles]; the
+ * indices hashed are the global ones, off_r + i); no counterpart in the reference */
+int32_t tissue_batch_set_visit_check(tissue_batch* b, int32_t enable);
+int32_t tissue_batch_get_visits(tissue_batch* b, int32_t* na, uint32_t* hash);
 
 /* ---- adjoint: gradients of a trajectory loss (no counterpart in the reference, which has no
  * differentiation; it differentiates n_steps of tissue_step) ----

@@ -267,3 +267,31 @@ def test_ensemble_workload_equals_solo_runs(T):
         assert len(ra) == steps and np.array_equal(ra, rb) and np.allclose(pa, pb, rtol=0, atol=1e-12), r
         s.close()
     b.close()
+
+
+@pytest.mark.parametrize("xcd", [0, 1])
+def test_batch_walk_visits_the_oracle_sets(T, O, xcd):
+    """The batch's own neighbour walk (visit check) at the batched launch shape, 2 lanes in
+    128-thread workgroups with the vertex-occupancy mask staged in LDS, in both workgroup orders:
+    every particle of every replica visits exactly its replica's oracle neighbour set (size and
+    id hash) at each of 8 steps."""
+    sigma = 0.05
+    m, tb = P.mesh_and_tables(32, sigma)
+    om = O.Mesh(tb)
+    ps, std = batch_params(T, sigma, len(BATCH_SIZES))
+    seeds = np.arange(len(BATCH_SIZES), dtype=np.uint64) + 40
+    b = T.Batch(m, BATCH_SIZES, ps)
+    b.set_mode(TILED, xcd)
+    b.set_launch(2, 128, 1)
+    b.set_visit_check(True)
+    b.init_random(seeds)
+    b.gen_noise(seeds, std, 8)
+    for t in range(8):
+        st = b.get_state()
+        assert b.run(1, t) == T.OK
+        na, h = b.visits()
+        for r, n in enumerate(BATCH_SIZES):
+            o = int(b.off[r])
+            ena, eh = P.visit_records(O.neighbor_sets_sparse(om, st.vid[o:o + n], sigma), off=o)
+            assert np.array_equal(na[o:o + n], ena) and np.array_equal(h[o:o + n], eh), (t, r)
+    b.close()
