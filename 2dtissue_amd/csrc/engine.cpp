@@ -131,6 +131,9 @@ void Engine::release() {
   dfree(d_tiles); dfree(d_tile_begin); dfree(partial); dfree(phi); dfree(noise); dfree(call_noise); dfree(d_row0);
   if (h_row0) (void)hipHostFree(h_row0);
   h_row0 = nullptr;
+  if (h_back) (void)hipHostFree(h_back);
+  h_back = nullptr;
+  h_back_cap = 0;
   adjoint_release();
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
@@ -592,12 +595,22 @@ int32_t Engine::run_once(const Shape& s, int n_steps, const float* nz, size_t st
   HIP_OK(hipEventRecord(ev0, stream));
   HIP_OK(hipGraphLaunch(it->second, stream));
   HIP_OK(hipEventRecord(ev1, stream));
-  std::vector<double> ph(need_phi);
-  std::vector<unsigned> fl(R);
-  std::vector<int> fb(R);
-  HIP_OK(hipMemcpyAsync(ph.data(), phi, need_phi * sizeof(double), hipMemcpyDeviceToHost, stream));
-  HIP_OK(hipMemcpyAsync(fl.data(), d_halt, R * sizeof(unsigned), hipMemcpyDeviceToHost, stream));
-  HIP_OK(hipMemcpyAsync(fb.data(), d_first_bad, R * sizeof(int), hipMemcpyDeviceToHost, stream));
+  // read back into pinned memory (direct copies; pageable ones are staged by the runtime)
+  const size_t back = need_phi * sizeof(double) + 2 * (size_t)R * sizeof(int);
+  if (h_back_cap < back) {
+    HIP_OK(hipStreamSynchronize(stream));
+    if (h_back) (void)hipHostFree(h_back);
+    h_back = nullptr;
+    h_back_cap = 0;
+    HIP_OK(hipHostMalloc((void**)&h_back, back, hipHostMallocDefault));
+    h_back_cap = back;
+  }
+  const double* ph = reinterpret_cast<const double*>(h_back);
+  const unsigned* fl = reinterpret_cast<const unsigned*>(h_back + need_phi * sizeof(double));
+  const int* fb = reinterpret_cast<const int*>(h_back + need_phi * sizeof(double) + (size_t)R * sizeof(int));
+  HIP_OK(hipMemcpyAsync((void*)ph, phi, need_phi * sizeof(double), hipMemcpyDeviceToHost, stream));
+  HIP_OK(hipMemcpyAsync((void*)fl, d_halt, R * sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+  HIP_OK(hipMemcpyAsync((void*)fb, d_first_bad, R * sizeof(int), hipMemcpyDeviceToHost, stream));
   HIP_OK(hipStreamSynchronize(stream));
   float ms = 0.f;
   HIP_OK(hipEventElapsedTime(&ms, ev0, ev1));

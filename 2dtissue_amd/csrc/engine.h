@@ -86,6 +86,8 @@ struct Engine {
   // before the run (numbering of recorded steps); uploaded from pinned host memory
   int* d_row0 = nullptr;
   int* h_row0 = nullptr;
+  unsigned char* h_back = nullptr;  // pinned read-back of a run: order history, flag words, first steps
+  size_t h_back_cap = 0;
   // per-step output stream (engine_record.cpp); null when not recording
   std::unique_ptr<struct Recorder> rec;
   void record_start(const char* dir, int every);
