@@ -46,11 +46,8 @@ int32_t status_from_flags(unsigned f) {
 Engine::Engine(tissue_mesh* m, int R_, const int* n_, const tissue_params* p_) {
   if (!m || !n_ || !p_ || R_ < 1) throw ApiError(TISSUE_E_INVALID_ARG, "bad engine arguments");
   try {
-    HIP_OK(hipGetDevice(&device));
-    mesh = m;
-    md = m->device_view();
+    // every argument is checked before the first device call
     R = R_;
-    resident_cap = resident_max_particles(md.nv);
     long tot = 0;
     for (int r = 0; r < R; ++r) {
       if (n_[r] < 1) throw ApiError(TISSUE_E_INVALID_ARG, "particle count must be positive");
@@ -64,6 +61,10 @@ Engine::Engine(tissue_mesh* m, int R_, const int* n_, const tissue_params* p_) {
     }
     if (tot > INT_MAX / 16) throw ApiError(TISSUE_E_INVALID_ARG, "too many particles");
     ntot = (int)tot;
+    HIP_OK(hipGetDevice(&device));
+    mesh = m;
+    md = m->device_view();
+    resident_cap = resident_max_particles(md.nv);
     // raw or sorted vertex buckets (raw_buckets_for); TISSUE_SORTED_BUCKETS = 1 / 0 forces the choice
     raw_buckets = raw_buckets_for(nmax, md.nv);
     if (const char* e = std::getenv("TISSUE_SORTED_BUCKETS")) {

@@ -146,3 +146,24 @@ def compare_step(O, om, p, s_in, s_out, out, nbrs_gpu, eta, sparse=False, phi_gp
         fails.update(f_phi)
         ref_o = dict(ref_o, phi_exempt=exempt)
     return fails, ref_o
+
+
+def teacher_forced(O, om, p, sim, noise, visits=False, sparse=False):
+    """Teacher-forced run of a Sim: len(noise) single steps, each recomputed by the oracle from
+    the GPU's own input state and compared with compare_step (with the step's own neighbour walk
+    when visits is set: the Sim must have its visit check on). Returns (failure counts summed over
+    the steps, particle-steps exempt from the phi_ref sign comparison)."""
+    total, exempt = {}, 0
+    for t in range(len(noise)):
+        s_in = sim.get_state()
+        nb = sim.neighbor_lists()
+        assert sim.step(1, noise[t]) == T().OK, t
+        fails, ref = compare_step(O, om, p, s_in, sim.get_state(), sim.get_outputs(), nb,
+                                  np.asarray(noise[t], np.float64), sparse, phi_gpu=last_phi(sim),
+                                  visits=sim.visits() if visits else None)
+        exempt += ref["phi_exempt"]
+        for k, v in fails.items():
+            total[k] = total.get(k, 0) + v
+        if any(fails.values()):
+            print(f"step {t}: {fails}", flush=True)
+    return total, exempt

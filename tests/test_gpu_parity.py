@@ -23,21 +23,7 @@ def run_teacher_forced(T, O, cfg, steps, lanes=0, mode=0, sparse=False, block=0)
         sim.set_visit_check(True)
     sim.init_random(cfg["seed"])
     noise = T.gen_noise(cfg["noise_seed"], 0.1, steps, cfg["n"])
-    total = {}
-    exempt = 0
-    for t in range(steps):
-        s_in = sim.get_state()
-        nb = sim.neighbor_lists()
-        assert sim.step(1, noise[t]) == T.OK
-        s_out = sim.get_state()
-        out = sim.get_outputs()
-        fails, ref = P.compare_step(O, om, p, s_in, s_out, out, nb, noise[t].astype(np.float64), sparse,
-                                    phi_gpu=P.last_phi(sim), visits=sim.visits() if check_visits else None)
-        exempt += ref["phi_exempt"]
-        for k, v in fails.items():
-            total[k] = total.get(k, 0) + v
-        if any(fails.values()):
-            print(f"step {t}: {fails}", flush=True)
+    total, exempt = P.teacher_forced(O, om, p, sim, noise, visits=check_visits, sparse=sparse)
     print(f"{cfg['n']} particles x {steps} steps: {exempt} particle-steps exempt from the phi_ref sign "
           f"comparison (|x vy - y vx| within its rounding bound)", flush=True)
     sim.close()

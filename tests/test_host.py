@@ -191,6 +191,27 @@ def test_mesh_errors(T):
     assert "host" in str(e.value)
 
 
+def test_empty_and_invalid_systems_are_rejected(T):
+    """Systems without particles, batches without replicas or with an empty replica, and
+    parameters out of range are refused with TISSUE_E_INVALID_ARG before any device call (so
+    this holds on a host without a GPU too)."""
+    m = T.Mesh.ellipsoid(2.0, 1.0, 1.0, 6)
+    m.build_tables(0.1, where=T.BUILD_HOST)
+    for n in (0, -3):
+        with pytest.raises(T.TissueError) as e:
+            T.Sim(m, T.params(sigma=0.1), n)
+        assert e.value.code == T.E_INVALID_ARG and "positive" in str(e.value)
+    with pytest.raises(T.TissueError) as e:
+        T.Batch(m, [], [])
+    assert e.value.code == T.E_INVALID_ARG
+    with pytest.raises(T.TissueError) as e:
+        T.Batch(m, [10, 0, 5], [T.params(sigma=0.1)] * 3)
+    assert e.value.code == T.E_INVALID_ARG and "positive" in str(e.value)
+    with pytest.raises(T.TissueError) as e:  # sigma above the one the table was built for
+        T.Batch(m, [10, 5], [T.params(sigma=0.1), T.params(sigma=0.2)])
+    assert e.value.code == T.E_INVALID_ARG
+
+
 def restated_init(T, tb, seed, n):
     """The area-uniform initialisation restated in numpy from its definition: cumulative FP64 3-D
     face areas, upper-bound face choice, folded barycentrics, heading 2 pi u wrapped to [-pi, pi),
