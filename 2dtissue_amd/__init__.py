@@ -169,7 +169,7 @@ SIGNATURES = [
                                          _D, _D]),
     ("tissue_gen_noise", C.c_int32, [C.c_uint64, C.c_double, C.c_int32, C.c_int32, _F]),
     ("tissue_philox4x32", None, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
-    ("tissue_seam_wrap", C.c_int32, [C.c_int32, _D, _D, _D, _I, C.c_int32, C.c_int32]),
+    ("tissue_seam_wrap", C.c_int32, [C.c_int32, _D, _D, _D, _I, C.c_int32, C.c_int32, C.c_int32]),
     ("tissue_random_init_host", C.c_int32, [_P, C.c_uint64, C.c_int32, _F, _F, _F]),
     ("tissue_record_start", C.c_int32, [_P, C.c_char_p, C.c_int32]),
     ("tissue_record_stop", C.c_int32, [_P]),
@@ -607,9 +607,10 @@ def gen_noise(seed: int, std: float, n_steps: int, n: int) -> np.ndarray:
     return out
 
 
-def seam_wrap(start, end, lanes: int = 1, max_crossings: int = 0):
+def seam_wrap(start, end, lanes: int = 1, max_crossings: int = 0, wave: bool = False):
     """The step's seam wrap of moves start[n, 2] -> end[n, 2] on the GPU
-    (EuclideanTiling::diagonal_seam_edges_square_border, EuclideanTiling.cpp:31-69).
+    (EuclideanTiling::diagonal_seam_edges_square_border, EuclideanTiling.cpp:31-69); wave selects
+    the loop of crowded systems (same results).
     Returns (status, wrapped end [n, 2], heading rotation [n] in radians, quarter turns [n])."""
     a = np.ascontiguousarray(start, np.float64).reshape(-1, 2)
     e = np.ascontiguousarray(end, np.float64).reshape(-1, 2).copy()
@@ -618,7 +619,7 @@ def seam_wrap(start, end, lanes: int = 1, max_crossings: int = 0):
         raise ValueError("start and end must have the same number of moves")
     d = np.zeros(n); q = np.zeros(n, np.int32)
     code = lib().tissue_seam_wrap(n, _ptr(a, C.c_double), _ptr(e, C.c_double), _ptr(d, C.c_double), _ptr(q, C.c_int32),
-                                  max_crossings, lanes)
+                                  max_crossings, lanes, 1 if wave else 0)
     if code not in (OK, E_TILING_NO_CONVERGE):
         _check(code)
     return code, e, d, q

@@ -221,7 +221,7 @@ int32_t tissue_random_init_host(const tissue_mesh* m, uint64_t seed, int32_t n, 
 }
 
 int32_t tissue_seam_wrap(int32_t n, const double* start_xy, double* end_xy, double* dtheta, int32_t* quarter,
-                         int32_t max_crossings, int32_t lanes) {
+                         int32_t max_crossings, int32_t lanes, int32_t wave) {
   return guarded([&] {
     if (n < 0 || (n > 0 && (!start_xy || !end_xy || !dtheta))) throw ApiError(TISSUE_E_INVALID_ARG, "bad seam-wrap arguments");
     if (!valid_lanes(lanes)) throw ApiError(TISSUE_E_INVALID_ARG, "lanes must be 1, 2, 4, 8, 16, 32 or 64");
@@ -243,7 +243,7 @@ int32_t tissue_seam_wrap(int32_t n, const double* start_xy, double* end_xy, doub
     HIP_OK(hipMemcpy(b.s, start_xy, 2 * sizeof(double) * (size_t)n, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(b.e, end_xy, 2 * sizeof(double) * (size_t)n, hipMemcpyHostToDevice));
     const int mc = max_crossings > 0 ? max_crossings : kDefaultMaxCrossings;
-    HIP_OK(tissue::launch_seam_wrap(lanes, n, b.s, b.e, b.d, b.q, b.ok, mc, nullptr));
+    HIP_OK(tissue::launch_seam_wrap(lanes, n, b.s, b.e, b.d, b.q, b.ok, mc, wave != 0 ? 1 : 0, nullptr));
     HIP_OK(hipStreamSynchronize(nullptr));
     std::vector<int> ok((size_t)n), qv((size_t)n);
     HIP_OK(hipMemcpy(end_xy, b.e, 2 * sizeof(double) * (size_t)n, hipMemcpyDeviceToHost));

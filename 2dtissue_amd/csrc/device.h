@@ -245,9 +245,10 @@ struct LdsSource {
 // (2DTissue.cpp:254-268). The force is summed in FP64 (its FP32 rounding feeds the FP32 speed
 // and Euler step; the FP64 value the large-displacement path); the alignment sums are FP32, redone
 // in FP64 when they nearly cancel.
-// Always inlined: as an out-of-line call (which the inliner picks once two kernels share an
-// instantiation) its result and live state go through scratch memory.
-template <int LANES, class Src>
+// WAVE selects the seam-wrap loop of crowded systems (seam_wrap_group). Always inlined: as an
+// out-of-line call (which the inliner picks once two kernels share an instantiation) its result
+// and live state go through scratch memory.
+template <int LANES, bool WAVE = false, class Src>
 __device__ __forceinline__ StepOut particle_step(const MeshDev& mesh, const StepParams& prm, const Src& src, int i, float4 me,
                                         float th, int v, float eta, int lane) {
   float Sx = 0.f, Sy = 0.f;
@@ -454,7 +455,7 @@ __device__ __forceinline__ StepOut particle_step(const MeshDev& mesh, const Step
   double px = xn, py = yn, dth = 0.0;
   const bool finite = isfinite(xn) && isfinite(yn) && isfinite(thn);
   if (!finite) o.fl |= kFlagNonfinite;
-  else if (!seam_wrap_group<LANES>(me.x, me.y, px, py, dth, prm.max_crossings, lane, &o.q)) o.fl |= kFlagTiling;
+  else if (!seam_wrap_group<LANES, WAVE>(me.x, me.y, px, py, dth, prm.max_crossings, lane, &o.q)) o.fl |= kFlagTiling;
   if (finite && !inside_unit(px, py)) o.fl |= kFlagLost;
   if (finite) thn = wrap_heading(thn + dth);
   o.xo = (float)px;

@@ -500,7 +500,7 @@ void Engine::enqueue(const Shape& s, int n_steps, const float* nz, size_t stride
         io.visit = d_visit;
         BucketSet out = bs[(bp + 1) % 3];
         if (raw_buckets) out.multi = nullptr;  // no finalize pass reads the sort list
-        HIP_OK(launch_step_tiled(s.lanes, s.block, false, md, d_reps, d_tiles, n_phys, bs[bp], out, bs[(bp + 2) % 3],
+        HIP_OK(launch_step_tiled(s.lanes, s.block, false, !raw_buckets, md, d_reps, d_tiles, n_phys, bs[bp], out, bs[(bp + 2) % 3],
                                  io, stream));
         if (rec)
           HIP_OK(launch_record(ntot, d_row0, t, rec->every, rec->depth, io.xycs_out, r3x, r3y, r3z, nbr, rec->d_ring,

@@ -211,7 +211,7 @@ int32_t Engine::adjoint(int T, const float* nz, size_t stride, int K, const tiss
       io.step = q;
       io.tape_fs = tp.fs + q * N;
       io.tape_q = tp.q + q * N;
-      HIP_OK(launch_step_tiled(s.lanes, s.block, true, md, d_reps, d_tiles, n_phys, segment_set(tp, kc, q),
+      HIP_OK(launch_step_tiled(s.lanes, s.block, true, !raw_buckets, md, d_reps, d_tiles, n_phys, segment_set(tp, kc, q),
                                segment_set(tp, kc, q + 1), segment_set(tp, kc, q + 2), io, rs));
       if (!raw_buckets)
         HIP_OK(launch_bucket_finalize(s.block, d_reps, d_tiles, n_phys, md.nv, R == 1 ? ntot : 0,

@@ -271,47 +271,229 @@ __host__ __device__ inline bool seam_wrap_flip(double ax, double ay, double& px,
   return ok;
 }
 
-// The same loop run by the LANES lanes of one particle (a lane group aligned within the wave,
-// all of its lanes active): the four border tests are spread over the lanes (lane % 4 tests
-// border lane % 4; with two lanes, lane k tests border k and border k + 2) and the first passing
-// border is read from a wave ballot. Same decisions and the same FP64 values as seam_wrap.
+// ---- the last flight of a wave, three crossings per pass
+//
+// Once a single flight of a wave still needs crossings, all 64 lanes advance it together, three
+// crossings per pass: lane l follows the path that crosses border (l >> 4) & 3 first, (l >> 2) & 3
+// second and l & 3 third, and before each of its crossings tests border l & 3 (border_pass) on
+// its path's current segment. The first passing border of each level is read from the ballots of
+// those tests (level 1: lanes 0..3; level 2: lanes 16 c1 + 0..3; level 3: lanes 16 c1 + 4 c2 +
+// 0..3), and the flight's state after the crossings taken is read from the lane whose path they
+// are. Every lane performs exactly the sequential loop's operations along its own path, so the
+// outcome is the sequential loop's bit for bit; a pass stops early where the flight lands inside
+// the square, reaches its crossing bound, or meets a level whose four tests all fail (the
+// reference's fall-through to the down mapping with the exit at the start), which every lane then
+// crosses alike.
+
+struct WrapState {
+  double ax, ay, px, py, dth;
+  int q;
+};
+
+// one crossing along border b (0..4) of the state, by cross_border_flip
+__host__ __device__ inline void wrap_cross(int b, WrapState& s) {
+  const double dx = s.px - s.ax, dy = s.py - s.ay;
+  cross_border_flip(b, s.ax, s.ay, s.px, s.py, dx, dy, s.dth, s.q);
+}
+
+// lane l's path from s: its three border tests and its states after one, two and three crossings
+struct WrapPath {
+  bool t1, t2, t3, in1, in2;
+  WrapState s1, s2, s3;
+};
+
+__host__ __device__ inline void wrap_path(int l, const WrapState& s, WrapPath& w) {
+  const int k = l & 3;
+  const bool lr = k < 2;
+  const double ca = k == 1 ? 1.0 : -0.0, cb = k == 2 ? 1.0 : -0.0;
+  w.t1 = border_pass(lr, ca, cb, s.ax, s.ay, s.px - s.ax, s.py - s.ay);
+  w.s1 = s;
+  wrap_cross((l >> 4) & 3, w.s1);
+  w.in1 = inside_unit(w.s1.px, w.s1.py);
+  w.t2 = border_pass(lr, ca, cb, w.s1.ax, w.s1.ay, w.s1.px - w.s1.ax, w.s1.py - w.s1.ay);
+  w.s2 = w.s1;
+  wrap_cross((l >> 2) & 3, w.s2);
+  w.in2 = inside_unit(w.s2.px, w.s2.py);
+  w.t3 = border_pass(lr, ca, cb, w.s2.ax, w.s2.ay, w.s2.px - w.s2.ax, w.s2.py - w.s2.ay);
+  w.s3 = w.s2;
+  wrap_cross(k, w.s3);
+}
+
+__host__ __device__ inline int first_of4(uint64_t bits) { return (bits & 0xFull) ? __builtin_ctzll(bits & 0xFull) : 4; }
+
+// The pass loop. Lanes supplies run(s) (every lane's wrap_path from the uniform state s, and the
+// 64-bit masks of t1, t2, t3, in1, in2 over the lanes) and take(level, lane) (lane's state after
+// level crossings). crossings counts as in seam_wrap; returns false at the crossing bound.
+struct WrapMasks {
+  uint64_t t1, t2, t3, in1, in2;
+};
+
+template <class Lanes>
+__host__ __device__ inline bool seam_wrap_passes(Lanes& L, WrapState& s, int& crossings, int max_crossings) {
+  for (;;) {
+    if (inside_unit(s.px, s.py)) return true;
+    if (crossings >= max_crossings) return false;
+    const WrapMasks m = L.run(s);
+    const int c1 = first_of4(m.t1);
+    ++crossings;
+    if (c1 == 4) {
+      wrap_cross(4, s);
+      continue;
+    }
+    int lane = c1 << 4;
+    if (((m.in1 >> lane) & 1) || crossings >= max_crossings) {
+      s = L.take(1, lane);
+      continue;
+    }
+    const int c2 = first_of4(m.t2 >> lane);
+    ++crossings;
+    if (c2 == 4) {
+      s = L.take(1, lane);
+      wrap_cross(4, s);
+      continue;
+    }
+    lane |= c2 << 2;
+    if (((m.in2 >> lane) & 1) || crossings >= max_crossings) {
+      s = L.take(2, lane);
+      continue;
+    }
+    const int c3 = first_of4(m.t3 >> lane);
+    ++crossings;
+    if (c3 == 4) {
+      s = L.take(2, lane);
+      wrap_cross(4, s);
+      continue;
+    }
+    s = L.take(3, lane | c3);
+  }
+}
+
+__device__ inline double readlane_f64(double x, int l) {
+  const uint64_t b = bits64(x);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), l);
+  return from_bits64(((uint64_t)hi << 32) | lo);
+}
+
+__device__ inline WrapState readlane_state(const WrapState& s, int l) {
+  return WrapState{readlane_f64(s.ax, l), readlane_f64(s.ay, l), readlane_f64(s.px, l), readlane_f64(s.py, l),
+                   readlane_f64(s.dth, l), __builtin_amdgcn_readlane(s.q, l)};
+}
+
+// the 64 lanes of a wave (all active) as seam_wrap_passes' Lanes
+struct WaveLanes {
+  WrapPath w;
+  __device__ WrapMasks run(const WrapState& s) {
+    wrap_path((int)__lane_id(), s, w);
+    return WrapMasks{__builtin_amdgcn_ballot_w64(w.t1), __builtin_amdgcn_ballot_w64(w.t2),
+                     __builtin_amdgcn_ballot_w64(w.t3), __builtin_amdgcn_ballot_w64(w.in1),
+                     __builtin_amdgcn_ballot_w64(w.in2)};
+  }
+  __device__ WrapState take(int level, int l) const {
+    return readlane_state(level == 1 ? w.s1 : (level == 2 ? w.s2 : w.s3), l);
+  }
+};
+
+// One crossing of the flight of a lane group of LANES lanes (aligned within the wave, all of its
+// lanes active): the four border tests are spread over the lanes (lane % 4 tests border lane % 4;
+// with two lanes, lane k tests border k and border k + 2; one lane tests all four) and the first
+// passing border is read from a wave ballot. k, ca, cb: the lane's border constants
+// (group_constants); base: the group's first lane.
 template <int LANES>
+__device__ inline void group_constants(int lane, int& k, double& ca, double& cb) {
+  k = LANES >= 4 ? (lane & 3) : (lane & 1);
+  ca = k == 1 ? 1.0 : -0.0;
+  cb = (LANES >= 4 ? k == 2 : k == 0) ? 1.0 : -0.0;
+}
+
+template <int LANES>
+__device__ inline void group_crossing(int k, double ca, double cb, unsigned base, double& ax, double& ay, double& px,
+                                      double& py, double& dtheta, int& quarter) {
+  const double dx = px - ax, dy = py - ay;
+  unsigned bits;
+  if constexpr (LANES >= 4) {
+    bits = (unsigned)(__builtin_amdgcn_ballot_w64(border_pass(k < 2, ca, cb, ax, ay, dx, dy)) >> base);
+  } else if constexpr (LANES == 2) {
+    const unsigned lo = (unsigned)(__builtin_amdgcn_ballot_w64(border_pass(true, ca, -0.0, ax, ay, dx, dy)) >> base) & 0x3u;
+    const unsigned hi = (unsigned)(__builtin_amdgcn_ballot_w64(border_pass(false, -0.0, cb, ax, ay, dx, dy)) >> base) & 0x3u;
+    bits = lo | (hi << 2);
+  } else {
+    (void)k; (void)ca; (void)cb; (void)base;
+    bits = 0u;
+    for (int b = 3; b >= 0; --b)
+      bits = (bits << 1) | (border_pass(b < 2, b == 1 ? 1.0 : -0.0, b == 2 ? 1.0 : -0.0, ax, ay, dx, dy) ? 1u : 0u);
+  }
+  // first passing border of the group (bits above the group's four belong to the next group: a
+  // lowest set bit at 4 or above, or none, means none of ours passed)
+  const unsigned low = (unsigned)__builtin_ctz(bits | 0x10u);
+  cross_border_flip((int)low, ax, ay, px, py, dx, dy, dtheta, quarter);
+}
+
+// The seam wrap run by the LANES lanes of one particle (group_crossing per crossing). Same
+// decisions and the same FP64 values as seam_wrap.
+// WAVE = false: each group loops on its own (finished groups wait masked).
+// WAVE = true (for crowded systems, whose longest flights run hundreds of crossings while the
+// other waves of the SIMD compete for issue): the groups of the wave stay in the loop together,
+// and when a full wave has a single flight left, its 64 lanes take it over three crossings per
+// pass (seam_wrap_passes); the flight's own lanes keep the result.
+template <int LANES, bool WAVE = false>
 __device__ inline bool seam_wrap_group(double ax, double ay, double& px, double& py, double& dtheta, int max_crossings,
                                        int lane, int* quarter_out) {
-  if constexpr (LANES == 1) {
-    (void)lane;
-    return seam_wrap(ax, ay, px, py, dtheta, max_crossings, nullptr, quarter_out);
-  } else {
-    const unsigned base = __lane_id() & ~(unsigned)(LANES - 1);
-    // lane constants of the lane's border(s): border k (four lanes or more), or borders k and
-    // k + 2 (two lanes), with ca, cb as border_pass takes them
-    const int k = LANES >= 4 ? (lane & 3) : (lane & 1);
-    const double ca = k == 1 ? 1.0 : -0.0;
-    const double cb = (LANES >= 4 ? k == 2 : k == 0) ? 1.0 : -0.0;
-    int crossings = 0;
-    int quarter = 0;
-    dtheta = 0.0;
-    bool ok = true;
+  const unsigned base = __lane_id() & ~(unsigned)(LANES - 1);
+  int k;
+  double ca, cb;
+  group_constants<LANES>(lane, k, ca, cb);
+  int crossings = 0;
+  int quarter = 0;
+  dtheta = 0.0;
+  bool ok = true;
+  if constexpr (!WAVE) {
     while (!inside_unit(px, py)) {
-      if (crossings >= max_crossings) { ok = false; break; }
-      ++crossings;
-      const double dx = px - ax, dy = py - ay;
-      unsigned bits;
-      if constexpr (LANES >= 4) {
-        bits = (unsigned)(__builtin_amdgcn_ballot_w64(border_pass(k < 2, ca, cb, ax, ay, dx, dy)) >> base);
-      } else {
-        const unsigned lo = (unsigned)(__builtin_amdgcn_ballot_w64(border_pass(true, ca, -0.0, ax, ay, dx, dy)) >> base) & 0x3u;
-        const unsigned hi = (unsigned)(__builtin_amdgcn_ballot_w64(border_pass(false, -0.0, cb, ax, ay, dx, dy)) >> base) & 0x3u;
-        bits = lo | (hi << 2);
+      if (crossings >= max_crossings) {
+        ok = false;
+        break;
       }
-      // first passing border of the group (bits above the group's four belong to the next group:
-      // a lowest set bit at 4 or above, or none, means none of ours passed)
-      const unsigned low = (unsigned)__builtin_ctz(bits | 0x10u);
-      cross_border_flip((int)low, ax, ay, px, py, dx, dy, dtheta, quarter);
+      ++crossings;
+      group_crossing<LANES>(k, ca, cb, base, ax, ay, px, py, dtheta, quarter);
     }
-    if (quarter_out) *quarter_out = quarter & 3;
-    return ok;
+  } else {
+    const bool full = __builtin_amdgcn_ballot_w64(true) == ~0ull;  // every lane of the wave is here
+    constexpr uint64_t kGroup = LANES == 64 ? ~0ull : ((1ull << LANES) - 1);
+    bool run = !inside_unit(px, py);  // the flight needs another crossing (bound not yet checked)
+    for (;;) {
+      const uint64_t act = __builtin_amdgcn_ballot_w64(run);
+      if (act == 0) break;
+      if (full) {
+        const int src = __builtin_ctzll(act);
+        if ((act & ~(kGroup << (src & ~(LANES - 1)))) == 0) {  // a single flight left
+          WrapState s = readlane_state(WrapState{ax, ay, px, py, dtheta, quarter}, src);
+          int c = __builtin_amdgcn_readlane(crossings, src);
+          WaveLanes wl;
+          const bool good = seam_wrap_passes(wl, s, c, max_crossings);
+          if (run) {
+            px = s.px;
+            py = s.py;
+            dtheta = s.dth;
+            quarter = s.q;
+            ok = good;
+          }
+          break;
+        }
+      }
+      if (run) {
+        if (crossings >= max_crossings) {
+          ok = false;
+          run = false;
+        } else {
+          ++crossings;
+          group_crossing<LANES>(k, ca, cb, base, ax, ay, px, py, dtheta, quarter);
+          run = !inside_unit(px, py);
+        }
+      }
+    }
   }
+  if (quarter_out) *quarter_out = quarter & 3;
+  return ok;
 }
 
 // heading kept in [-pi, pi)

@@ -57,7 +57,7 @@ __device__ __forceinline__ void wave_order_partial(int sgn, long long cs, long l
   if ((threadIdx.x & 63) == 0) *out = OrderPartial{cs, sn, sgn, 0};
 }
 
-template <int LANES, bool TAPE, bool OCC, bool VISITS>
+template <int LANES, bool TAPE, bool OCC, bool VISITS, bool WAVE>
 __global__ void __launch_bounds__(kStepMaxBlock) k_step_tiled(MeshDev mesh, const ReplicaDesc* reps,
                                                               const TileDesc* tiles, BucketSet bin, BucketSet bout,
                                                               BucketSet bclr, TiledIO io) {
@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(kStepMaxBlock) k_step_tiled(MeshDev mesh, cons
       }
     } else {
       const GlobalSource<OCC> src{replica_in(bin, r, nv, rep.off), io.xycs_in, io.th_in, s_occ};
-      const StepOut o = particle_step<LANES>(mesh, rep.prm, src, g, me, th, v, eta, lane);
+      const StepOut o = particle_step<LANES, WAVE>(mesh, rep.prm, src, g, me, th, v, eta, lane);
       if (lane == 0) {
         io.xycs_out[g] = make_float4(o.xo, o.yo, o.co, o.so);
         io.th_out[g] = o.to;
@@ -335,7 +335,7 @@ __global__ void k_prepare(int n, const float* x, const float* y, const float* th
 }
 
 // seam wrap of n moves, LANES lanes per move (the step kernel's lane groups); whole groups only
-template <int LANES>
+template <int LANES, bool WAVE>
 __global__ void __launch_bounds__(256) k_seam_wrap(int n, const double2* start, double2* end, double* dtheta,
                                                    int* quarter, int* ok, int max_crossings) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -344,7 +344,7 @@ __global__ void __launch_bounds__(256) k_seam_wrap(int n, const double2* start, 
   const double2 a = start[i];
   double px = end[i].x, py = end[i].y, dth = 0.0;
   int q = 0;
-  const bool good = seam_wrap_group<LANES>(a.x, a.y, px, py, dth, max_crossings, lane, &q);
+  const bool good = seam_wrap_group<LANES, WAVE>(a.x, a.y, px, py, dth, max_crossings, lane, &q);
   if (lane == 0) {
     end[i] = make_double2(px, py);
     dtheta[i] = dth;
@@ -427,46 +427,62 @@ __global__ void k_neighbor_lists(MeshDev mesh, const ReplicaDesc* reps, const in
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
-template <int LANES, bool OCC>
-void launch_tiled_variant(int block, bool tape, const MeshDev& mesh, const ReplicaDesc* reps, const TileDesc* tiles,
-                          int n_tiles, const BucketSet& bin, const BucketSet& bout, const BucketSet& bclr,
-                          const TiledIO& io, hipStream_t st) {
+template <int LANES, bool OCC, bool WAVE>
+void launch_tiled_mode(int block, bool tape, const MeshDev& mesh, const ReplicaDesc* reps, const TileDesc* tiles,
+                       int n_tiles, const BucketSet& bin, const BucketSet& bout, const BucketSet& bclr,
+                       const TiledIO& io, hipStream_t st) {
   const size_t lds = OCC ? sizeof(unsigned) * (size_t)occ_words(mesh.nv) : 0;
   if (tape)
-    hipLaunchKernelGGL((k_step_tiled<LANES, true, OCC, false>), dim3(n_tiles), dim3(block), lds, st, mesh, reps, tiles,
-                       bin, bout, bclr, io);
+    hipLaunchKernelGGL((k_step_tiled<LANES, true, OCC, false, WAVE>), dim3(n_tiles), dim3(block), lds, st, mesh, reps,
+                       tiles, bin, bout, bclr, io);
   else if (io.visit)  // the visit check: the same step, recording each particle's visited set
-    hipLaunchKernelGGL((k_step_tiled<LANES, false, OCC, true>), dim3(n_tiles), dim3(block), lds, st, mesh, reps, tiles,
-                       bin, bout, bclr, io);
+    hipLaunchKernelGGL((k_step_tiled<LANES, false, OCC, true, WAVE>), dim3(n_tiles), dim3(block), lds, st, mesh, reps,
+                       tiles, bin, bout, bclr, io);
   else
-    hipLaunchKernelGGL((k_step_tiled<LANES, false, OCC, false>), dim3(n_tiles), dim3(block), lds, st, mesh, reps,
+    hipLaunchKernelGGL((k_step_tiled<LANES, false, OCC, false, WAVE>), dim3(n_tiles), dim3(block), lds, st, mesh, reps,
                        tiles, bin, bout, bclr, io);
 }
 
+// wave: the seam-wrap loop of crowded systems (seam_wrap_group), compiled for up to 8 lanes per
+// particle (the shapes such systems run at); wider shapes take the plain loop (same results)
+template <int LANES, bool OCC>
+void launch_tiled_variant(int block, bool tape, bool wave, const MeshDev& mesh, const ReplicaDesc* reps,
+                          const TileDesc* tiles, int n_tiles, const BucketSet& bin, const BucketSet& bout,
+                          const BucketSet& bclr, const TiledIO& io, hipStream_t st) {
+  if constexpr (LANES <= 8) {
+    if (wave) {
+      launch_tiled_mode<LANES, OCC, true>(block, tape, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st);
+      return;
+    }
+  }
+  (void)wave;
+  launch_tiled_mode<LANES, OCC, false>(block, tape, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st);
+}
+
 template <int LANES>
-void launch_tiled_lanes(int block, bool tape, const MeshDev& mesh, const ReplicaDesc* reps, const TileDesc* tiles,
-                        int n_tiles, const BucketSet& bin, const BucketSet& bout, const BucketSet& bclr,
-                        const TiledIO& io, hipStream_t st) {
+void launch_tiled_lanes(int block, bool tape, bool wave, const MeshDev& mesh, const ReplicaDesc* reps,
+                        const TileDesc* tiles, int n_tiles, const BucketSet& bin, const BucketSet& bout,
+                        const BucketSet& bclr, const TiledIO& io, hipStream_t st) {
   if (stage_occupancy(mesh.nv, block, LANES))
-    launch_tiled_variant<LANES, true>(block, tape, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st);
+    launch_tiled_variant<LANES, true>(block, tape, wave, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st);
   else
-    launch_tiled_variant<LANES, false>(block, tape, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st);
+    launch_tiled_variant<LANES, false>(block, tape, wave, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st);
 }
 
 }  // namespace
 
-hipError_t launch_step_tiled(int lanes, int block, bool tape, const MeshDev& mesh, const ReplicaDesc* reps,
+hipError_t launch_step_tiled(int lanes, int block, bool tape, bool wave, const MeshDev& mesh, const ReplicaDesc* reps,
                              const TileDesc* tiles, int n_tiles, const BucketSet& bin, const BucketSet& bout,
                              const BucketSet& bclr, const TiledIO& io, hipStream_t st) {
   if (block < 64 || block > kStepMaxBlock || block % 64 || n_tiles < 1) return hipErrorInvalidValue;
   switch (lanes) {
-    case 1: launch_tiled_lanes<1>(block, tape, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st); break;
-    case 2: launch_tiled_lanes<2>(block, tape, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st); break;
-    case 4: launch_tiled_lanes<4>(block, tape, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st); break;
-    case 8: launch_tiled_lanes<8>(block, tape, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st); break;
-    case 16: launch_tiled_lanes<16>(block, tape, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st); break;
-    case 32: launch_tiled_lanes<32>(block, tape, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st); break;
-    case 64: launch_tiled_lanes<64>(block, tape, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st); break;
+    case 1: launch_tiled_lanes<1>(block, tape, wave, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st); break;
+    case 2: launch_tiled_lanes<2>(block, tape, wave, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st); break;
+    case 4: launch_tiled_lanes<4>(block, tape, wave, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st); break;
+    case 8: launch_tiled_lanes<8>(block, tape, wave, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st); break;
+    case 16: launch_tiled_lanes<16>(block, tape, wave, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st); break;
+    case 32: launch_tiled_lanes<32>(block, tape, wave, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st); break;
+    case 64: launch_tiled_lanes<64>(block, tape, wave, mesh, reps, tiles, n_tiles, bin, bout, bclr, io, st); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -508,19 +524,29 @@ hipError_t launch_locate(const MeshDev& mesh, int n, const float4* xycs, int* vi
   return hipGetLastError();
 }
 
-hipError_t launch_seam_wrap(int lanes, int n, const double2* start, double2* end, double* dtheta, int* quarter, int* ok,
-                            int max_crossings, hipStream_t st) {
-  const dim3 grid(cdiv(n * lanes, 256)), block(256);
+template <bool WAVE>
+void launch_seam_wrap_mode(int lanes, dim3 grid, dim3 block, int n, const double2* start, double2* end, double* dtheta,
+                           int* quarter, int* ok, int max_crossings, hipStream_t st) {
   switch (lanes) {
-    case 1: hipLaunchKernelGGL(k_seam_wrap<1>, grid, block, 0, st, n, start, end, dtheta, quarter, ok, max_crossings); break;
-    case 2: hipLaunchKernelGGL(k_seam_wrap<2>, grid, block, 0, st, n, start, end, dtheta, quarter, ok, max_crossings); break;
-    case 4: hipLaunchKernelGGL(k_seam_wrap<4>, grid, block, 0, st, n, start, end, dtheta, quarter, ok, max_crossings); break;
-    case 8: hipLaunchKernelGGL(k_seam_wrap<8>, grid, block, 0, st, n, start, end, dtheta, quarter, ok, max_crossings); break;
-    case 16: hipLaunchKernelGGL(k_seam_wrap<16>, grid, block, 0, st, n, start, end, dtheta, quarter, ok, max_crossings); break;
-    case 32: hipLaunchKernelGGL(k_seam_wrap<32>, grid, block, 0, st, n, start, end, dtheta, quarter, ok, max_crossings); break;
-    case 64: hipLaunchKernelGGL(k_seam_wrap<64>, grid, block, 0, st, n, start, end, dtheta, quarter, ok, max_crossings); break;
-    default: return hipErrorInvalidValue;
+    case 1: hipLaunchKernelGGL((k_seam_wrap<1, WAVE>), grid, block, 0, st, n, start, end, dtheta, quarter, ok, max_crossings); break;
+    case 2: hipLaunchKernelGGL((k_seam_wrap<2, WAVE>), grid, block, 0, st, n, start, end, dtheta, quarter, ok, max_crossings); break;
+    case 4: hipLaunchKernelGGL((k_seam_wrap<4, WAVE>), grid, block, 0, st, n, start, end, dtheta, quarter, ok, max_crossings); break;
+    case 8: hipLaunchKernelGGL((k_seam_wrap<8, WAVE>), grid, block, 0, st, n, start, end, dtheta, quarter, ok, max_crossings); break;
+    case 16: hipLaunchKernelGGL((k_seam_wrap<16, WAVE>), grid, block, 0, st, n, start, end, dtheta, quarter, ok, max_crossings); break;
+    case 32: hipLaunchKernelGGL((k_seam_wrap<32, WAVE>), grid, block, 0, st, n, start, end, dtheta, quarter, ok, max_crossings); break;
+    case 64: hipLaunchKernelGGL((k_seam_wrap<64, WAVE>), grid, block, 0, st, n, start, end, dtheta, quarter, ok, max_crossings); break;
+    default: break;
   }
+}
+
+hipError_t launch_seam_wrap(int lanes, int n, const double2* start, double2* end, double* dtheta, int* quarter, int* ok,
+                            int max_crossings, int wave, hipStream_t st) {
+  if (lanes < 1 || lanes > 64 || (lanes & (lanes - 1))) return hipErrorInvalidValue;
+  const dim3 grid(cdiv(n * lanes, 256)), block(256);
+  if (wave)
+    launch_seam_wrap_mode<true>(lanes, grid, block, n, start, end, dtheta, quarter, ok, max_crossings, st);
+  else
+    launch_seam_wrap_mode<false>(lanes, grid, block, n, start, end, dtheta, quarter, ok, max_crossings, st);
   return hipGetLastError();
 }
 

@@ -162,8 +162,8 @@ hipError_t launch_resident(int lanes, int block, const tdev::MeshDev& mesh, cons
 
 // one step of every replica: n_tiles workgroups of `block` threads, `lanes` lanes per particle;
 // reads bucket set bin, writes bout, clears bclr; tape selects the variant that records the
-// adjoint tape
-hipError_t launch_step_tiled(int lanes, int block, bool tape, const tdev::MeshDev& mesh, const ReplicaDesc* reps,
+// adjoint tape; wave the seam-wrap loop of crowded systems (same results, see seam_wrap_group)
+hipError_t launch_step_tiled(int lanes, int block, bool tape, bool wave, const tdev::MeshDev& mesh, const ReplicaDesc* reps,
                              const TileDesc* tiles, int n_tiles, const BucketSet& bin, const BucketSet& bout,
                              const BucketSet& bclr, const TiledIO& io, hipStream_t st);
 // sort the buckets of set bs that hold two or more particles (over the same tiles as the step);
@@ -194,9 +194,10 @@ hipError_t launch_record(int ntot, const int* ctl, int t, int every, int depth, 
 // area-uniform initial positions and headings of every replica (replica r from seeds[r])
 hipError_t launch_init_random(int ntot, const int* rid, const ReplicaDesc* reps, const uint64_t* seeds,
                               const double* cdf, const tdev::MeshDev& mesh, float* x, float* y, float* th, hipStream_t st);
-// seam wrap of n moves start -> end (end overwritten), lanes = 1, 2, 4, 8 or 16 per move
+// seam wrap of n moves start -> end (end overwritten), lanes = 1, 2, 4, 8, 16, 32 or 64 per move;
+// wave: a wave's last flight is finished by all of its lanes (seam_wrap_group)
 hipError_t launch_seam_wrap(int lanes, int n, const double2* start, double2* end, double* dtheta, int* quarter, int* ok,
-                            int max_crossings, hipStream_t st);
+                            int max_crossings, int wave, hipStream_t st);
 // bucket every particle of every replica (rid[g] = replica of particle g) into set `bs`
 hipError_t launch_insert(int ntot, const int* vid, const int* rid, const ReplicaDesc* reps, int nv,
                          const BucketSet& bs, hipStream_t st);

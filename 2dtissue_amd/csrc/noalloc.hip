@@ -273,7 +273,7 @@ extern "C" int32_t tissue_step_noalloc(const tissue_mesh* mc, const tissue_param
     io.halt_clr = a.rows + 2;
     io.first_bad = a.first_bad;
     io.step = 0;
-    HIP_OK(tissue::launch_step_tiled(lanes, block, false, md, a.rep, a.tiles, n_tiles, b[0], b[1], b[2], io, st));
+    HIP_OK(tissue::launch_step_tiled(lanes, block, false, sorted != 0, md, a.rep, a.tiles, n_tiles, b[0], b[1], b[2], io, st));
     HIP_OK(tissue::launch_phi_tiles(1, 1, a.tile_begin, n_tiles, tissue::partials_per_tile(block), io.partial, a.rep,
                                     phi, 1, 0, st));
     hipLaunchKernelGGL(k_noalloc_finish, dim3(cdiv(n, 256)), dim3(256), 0, st, n, io.xycs_out, x_out, y_out,

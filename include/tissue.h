@@ -252,11 +252,13 @@ int32_t tissue_record_stop(tissue_sim* s);
  * -> r_UV: start_xy[2n] are the positions before the move (in the unit square), end_xy[2n] the
  * positions after it, overwritten with the wrapped positions; dtheta[n] receives the heading
  * rotation (radians), quarter[n] (may be NULL) the quarter turns of the composed map. lanes (1, 2,
- * 4, 8, 16) is the number of threads per move, as in the step kernel; max_crossings <= 0 selects
- * 16384. Returns TISSUE_E_TILING_NO_CONVERGE when a move needs more crossings (its outputs are
- * those reached at the bound). Host arrays; runs synchronously on the current device. */
+ * 4, 8, 16, 32, 64) is the number of threads per move, as in the step kernel; max_crossings <= 0
+ * selects 16384; wave = 1 runs the loop of crowded systems (a wave's last unfinished move is taken
+ * over by all 64 lanes of the wave, three crossings per pass), 0 the plain loop: same results.
+ * Returns TISSUE_E_TILING_NO_CONVERGE when a move needs more crossings (its outputs are those
+ * reached at the bound). Host arrays; runs synchronously on the current device. */
 int32_t tissue_seam_wrap(int32_t n, const double* start_xy, double* end_xy, double* dtheta, int32_t* quarter,
-                         int32_t max_crossings, int32_t lanes);
+                         int32_t max_crossings, int32_t lanes, int32_t wave);
 
 /* ---- one step without allocation ----
  * No counterpart in the reference. For callers that cannot allocate device memory: code under

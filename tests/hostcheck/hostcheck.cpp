@@ -49,6 +49,57 @@ void hc_seam_wrap_flip_many(int n, const double* a, const double* p, int max_cro
   }
 }
 
+// the wave's three-crossings-per-pass loop (seam_wrap_passes) with its 64 lanes emulated on the
+// host, entered after `before` crossings of the lane-group loop (the lane-parallel form above);
+// nones (may be null) counts, per move, the crossings of that first part whose four tests failed
+struct HostLanes {
+  tdev::WrapPath w[64];
+  tdev::WrapMasks run(const tdev::WrapState& s) {
+    tdev::WrapMasks m{0, 0, 0, 0, 0};
+    for (int l = 0; l < 64; ++l) {
+      tdev::wrap_path(l, s, w[l]);
+      m.t1 |= (uint64_t)w[l].t1 << l;
+      m.t2 |= (uint64_t)w[l].t2 << l;
+      m.t3 |= (uint64_t)w[l].t3 << l;
+      m.in1 |= (uint64_t)w[l].in1 << l;
+      m.in2 |= (uint64_t)w[l].in2 << l;
+    }
+    return m;
+  }
+  tdev::WrapState take(int level, int l) const { return level == 1 ? w[l].s1 : (level == 2 ? w[l].s2 : w[l].s3); }
+};
+
+void hc_seam_wrap_passes_many(int n, const double* a, const double* p, int max_crossings, int before, double* out_p,
+                              double* dth, int* ok, int* crossings, int* quarter, int* nones) {
+  HostLanes L;
+  for (int i = 0; i < n; ++i) {
+    tdev::WrapState s{a[2 * i], a[2 * i + 1], p[2 * i], p[2 * i + 1], 0.0, 0};
+    int c = 0;
+    bool good = true;
+    // the lane-group loop's crossings first (cross_border_flip after border_pass, as on the GPU)
+    while (c < before && !tdev::inside_unit(s.px, s.py)) {
+      if (c >= max_crossings) {
+        good = false;
+        break;
+      }
+      ++c;
+      const double dx = s.px - s.ax, dy = s.py - s.ay;
+      int b = 4;
+      for (int k = 3; k >= 0; --k)
+        if (tdev::border_pass(k < 2, k == 1 ? 1.0 : -0.0, k == 2 ? 1.0 : -0.0, s.ax, s.ay, dx, dy)) b = k;
+      if (nones && b == 4) ++nones[i];
+      tdev::cross_border_flip(b, s.ax, s.ay, s.px, s.py, dx, dy, s.dth, s.q);
+    }
+    if (good) good = tdev::seam_wrap_passes(L, s, c, max_crossings);
+    out_p[2 * i] = s.px;
+    out_p[2 * i + 1] = s.py;
+    dth[i] = s.dth;
+    ok[i] = good ? 1 : 0;
+    crossings[i] = c;
+    quarter[i] = s.q & 3;
+  }
+}
+
 void hc_seam_wrap_quarter_many(int n, const double* a, const double* p, int max_crossings, int* quarter) {
   for (int i = 0; i < n; ++i) {
     double px = p[2 * i], py = p[2 * i + 1], dt = 0.0;

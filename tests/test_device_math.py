@@ -146,6 +146,61 @@ def test_seam_wrap_flip_form_bitwise(H):
     assert c1.max() > 40 and (c1 > 0).sum() > 10000
 
 
+def wrap_passes(H, a, b, maxc, before):
+    n = len(a)
+    a = np.ascontiguousarray(a, np.float64); b = np.ascontiguousarray(b, np.float64)
+    out = np.zeros((n, 2)); dth = np.zeros(n); ok = np.zeros(n, np.int32); cr = np.zeros(n, np.int32)
+    q = np.zeros(n, np.int32); nones = np.zeros(n, np.int32)
+    H.hc_seam_wrap_passes_many(n, p(a, C.c_double), p(b, C.c_double), maxc, before, p(out, C.c_double),
+                               p(dth, C.c_double), p(ok, C.c_int32), p(cr, C.c_int32), p(q, C.c_int32),
+                               p(nones, C.c_int32))
+    return out, dth, ok, cr, q, nones
+
+
+@pytest.mark.parametrize("maxc", [1, 2, 3, 4, 5, 7, 4096])
+def test_seam_wrap_passes_bitwise(H, maxc):
+    """The last flight of a wave, three crossings per pass by 64 speculating lanes (host emulation
+    of the GPU's loop, entered after 0, 1, 2 or 5 ordinary crossings), against the sequential
+    loop, bit for bit: same crossing counts, end points, rotations and quarter turns, with the
+    crossing bound met at every level of a pass; includes flights whose tests all fail at some
+    crossing (the fall-through to the down mapping)."""
+    rng = np.random.default_rng(17)
+    n = 6000
+    a = rng.random((n, 2))
+    k = np.arange(n) % 8
+    a[k == 1, 0] = 0.0
+    a[k == 2, 1] = 1.0
+    a[k == 3] = 0.0
+    a[k == 4] = (1.0, 0.0)
+    L = np.exp(rng.uniform(np.log(1e-3), np.log(120.0), n))
+    ang = rng.uniform(-np.pi, np.pi, n)
+    b = a + L[:, None] * np.stack([np.cos(ang), np.sin(ang)], 1)
+    m = rng.random(n) < 0.1
+    b[m, rng.integers(0, 2)] = rng.choice([2.0, -1.0, 1.0, -0.0, 3.0], m.sum())
+    c = rng.choice([0.0, 1.0], (2000, 2))
+    off = rng.choice([1e-3, 9.99e-4, 1.001e-3, 1e-6, 0.0], (2000, 2)) * rng.choice([-1.0, 1.0], (2000, 2))
+    st = rng.random((2000, 2))
+    b2 = st + ((c + off) - st) * rng.uniform(1.0, 30.0, (2000, 1))
+    b2[:300, 0] = st[:300, 0]
+    A = np.concatenate([a, st]); B = np.concatenate([b, b2])
+    keep = ~(((B >= 0) & (B <= 1)).all(1))
+    A, B = A[keep], B[keep]
+    (o1, t1, k1, c1, q1), _ = wrap_flip_vs_direct(H, A, B, maxc)
+    for before in (0, 1, 2, 5):
+        o2, t2, k2, c2, q2, _ = wrap_passes(H, A, B, maxc, before)
+        assert np.array_equal(k1, k2) and np.array_equal(c1, c2) and np.array_equal(q1, q2), before
+        assert np.array_equal(o1.view(np.uint64), o2.view(np.uint64)), before
+        assert np.array_equal(t1.view(np.uint64), t2.view(np.uint64)), before
+    if maxc == 4096:
+        assert c1.max() > 100
+        # crossings whose four tests all fail (counted by the sequential loop) occur mid-flight
+        nones = wrap_passes(H, A, B, maxc, 1 << 30)[5]
+        print("moves with a fall-through crossing:", int((nones > 0).sum()), flush=True)
+        assert (nones > 0).sum() >= 10
+    else:
+        assert (k1 == 0).sum() > 100  # the bound is met
+
+
 def test_seam_wrap_reference_vector(H, O):
     import json
     g = json.load(open(os.path.join(HERE, "golden", "reference_vectors.json")))["seam_wrap"]

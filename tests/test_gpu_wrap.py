@@ -2,13 +2,15 @@
 restatement of EuclideanTiling::diagonal_seam_edges_square_border (EuclideanTiling.cpp:31-69), bit
 for bit, for every lane-group width the step kernel uses: long random flights (hundreds of
 crossings), moves that start on edges and corners, and lines passing within about 1e-3 of a corner
-(where the exit point snaps)."""
+(where the exit point snaps). Both loops are covered: the plain one, and the one of crowded
+systems, where every wave's last flight is finished by the wave's 64 lanes three crossings per
+pass, entered after however many crossings the other flights of the wave took."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
-LANES = (1, 2, 4, 8, 16)
+LANES = (1, 2, 4, 8, 16, 32, 64)
 
 
 def flights(seed=11, n=8000):
@@ -47,10 +49,11 @@ def cases(O):
     return out
 
 
+@pytest.mark.parametrize("wave", [False, True])
 @pytest.mark.parametrize("lanes", LANES)
-def test_seam_wrap_gpu_matches_oracle(T, cases, lanes):
+def test_seam_wrap_gpu_matches_oracle(T, cases, lanes, wave):
     for name, a, b, r, h, cr in cases:
-        code, out, dth, q = T.seam_wrap(a, b, lanes=lanes)
+        code, out, dth, q = T.seam_wrap(a, b, lanes=lanes, wave=wave)
         assert code == T.OK, name
         assert np.array_equal(out, r), name    # same crossings, same FP64 images
         assert np.array_equal(dth, h), name    # same rotations, accumulated in the same order
@@ -60,5 +63,22 @@ def test_seam_wrap_gpu_matches_oracle(T, cases, lanes):
 
 def test_seam_wrap_gpu_bound(T):
     # a flight that needs more crossings than allowed reports it
-    code, out, dth, q = T.seam_wrap([[0.5, 0.5]], [[300.5, 120.5]], lanes=4, max_crossings=10)
-    assert code == T.E_TILING_NO_CONVERGE
+    for wave in (False, True):
+        code, out, dth, q = T.seam_wrap([[0.5, 0.5]], [[300.5, 120.5]], lanes=4, max_crossings=10, wave=wave)
+        assert code == T.E_TILING_NO_CONVERGE
+
+
+@pytest.mark.parametrize("wave", [False, True])
+@pytest.mark.parametrize("lanes", [1, 16, 64])
+def test_seam_wrap_gpu_bound_at_every_level(T, O, cases, lanes, wave):
+    """Crossing bounds of 1 to 7 end flights at every level of the three-crossing passes: a move
+    fails exactly when the oracle needs more crossings than allowed, and the others equal the
+    oracle's bit for bit."""
+    name, a, b, r, h, cr = cases[0]
+    for maxc in range(1, 8):
+        code, out, dth, q = T.seam_wrap(a, b, lanes=lanes, max_crossings=maxc, wave=wave)
+        bad = cr > maxc
+        assert (code == T.E_TILING_NO_CONVERGE) == bool(bad.any()), maxc
+        good = ~bad
+        assert np.array_equal(out[good], r[good]), maxc
+        assert np.array_equal(dth[good], h[good]), maxc
