@@ -417,11 +417,15 @@ __device__ inline void group_crossing(int k, double ca, double cb, unsigned base
     const unsigned lo = (unsigned)(__builtin_amdgcn_ballot_w64(border_pass(true, ca, -0.0, ax, ay, dx, dy)) >> base) & 0x3u;
     const unsigned hi = (unsigned)(__builtin_amdgcn_ballot_w64(border_pass(false, -0.0, cb, ax, ay, dx, dy)) >> base) & 0x3u;
     bits = lo | (hi << 2);
-  } else {
+  } else {  // one lane: seam_wrap's crossing (the four tests side by side, reciprocals started early)
     (void)k; (void)ca; (void)cb; (void)base;
-    bits = 0u;
-    for (int b = 3; b >= 0; --b)
-      bits = (bits << 1) | (border_pass(b < 2, b == 1 ? 1.0 : -0.0, b == 2 ? 1.0 : -0.0, ax, ay, dx, dy) ? 1u : 0u);
+    const double omx = 1.0 - ax, omy = 1.0 - ay;
+    const double rx = quotient_reciprocal(dx), ry = quotient_reciprocal(-dy);
+    const int b4 = border_test(0, ax, ay, dx, dy, omx, omy) | (border_test(1, ax, ay, dx, dy, omx, omy) << 1) |
+                   (border_test(2, ax, ay, dx, dy, omx, omy) << 2) | (border_test(3, ax, ay, dx, dy, omx, omy) << 3);
+    const int border = (b4 & 1) ? 0 : ((b4 & 2) ? 1 : ((b4 & 4) ? 2 : ((b4 & 8) ? 3 : 4)));
+    cross_border(border, ax, ay, px, py, dx, dy, omx, omy, rx, ry, dtheta, quarter);
+    return;
   }
   // first passing border of the group (bits above the group's four belong to the next group: a
   // lowest set bit at 4 or above, or none, means none of ours passed)
@@ -447,7 +451,10 @@ __device__ inline bool seam_wrap_group(double ax, double ay, double& px, double&
   int quarter = 0;
   dtheta = 0.0;
   bool ok = true;
-  if constexpr (!WAVE) {
+  if constexpr (!WAVE && LANES == 1) {
+    (void)lane;
+    return seam_wrap(ax, ay, px, py, dtheta, max_crossings, nullptr, quarter_out);
+  } else if constexpr (!WAVE) {
     while (!inside_unit(px, py)) {
       if (crossings >= max_crossings) {
         ok = false;
