@@ -281,6 +281,54 @@ def batched(T, mesh, steps, n_replicas=256, plain_sample=16):
     return out
 
 
+def batched_sweep(T, mesh, steps, n_replicas=256, n=200):
+    """A parameter sweep: 256 replicas of 200 particles each with the parameter spread of the
+    batched workload (parameter seed 3, init seed 2000 + r, noise seed 1000 + r), `steps` steps.
+    The autotuner's choice is timed, and beside it the persistent LDS-resident kernel and the tiled
+    step kernel, each at the best of three launch shapes. One equal replica per CU is where the
+    persistent kernel (a workgroup per replica, all steps of a launch in LDS) wins; with uneven
+    sizes the largest replica's workgroup sets its pace and the tiled kernel wins."""
+    w = WORKLOAD
+    sizes = np.full(n_replicas, n)
+    rng = np.random.default_rng(3)
+    tau = np.exp(rng.uniform(np.log(0.1), np.log(10.0), n_replicas))
+    std = rng.uniform(0.0, 0.5, n_replicas)
+    v0 = rng.uniform(0.5, 2.0, n_replicas)
+    ps = [T.params(sigma=w["sigma"], v0=float(v0[r]), k=w["k"], mu=w["mu"], tau=float(tau[r]), dt=w["dt"])
+          for r in range(n_replicas)]
+    init_seeds = np.arange(n_replicas, dtype=np.uint64) + 2000
+    noise_seeds = np.arange(n_replicas, dtype=np.uint64) + 1000
+    b = T.Batch(mesh, sizes, ps)
+    b.init_random(init_seeds)
+    b.gen_noise(noise_seeds, std, steps)
+
+    def timed():
+        warm_graph(b.run, steps, 0)
+        b.init_random(init_seeds)
+        t0 = time.perf_counter()
+        code = b.run(steps, 0)
+        return code, b.ntot * steps / (time.perf_counter() - t0), b.last_run_ms() * 1e3 / steps, b.launch()
+
+    log("batched, parameter sweep of equal replicas: autotuning")
+    b.autotune(20)
+    code, value, us, shape = timed()
+    per_mode = {}
+    for mode, name in ((T.MODE_RESIDENT, "resident"), (T.MODE_STEP_KERNEL, "tiled")):
+        best = None
+        for lanes, block in ((1, 512), (2, 512), (4, 512)) if mode == T.MODE_RESIDENT else ((2, 128), (4, 128), (8, 128)):
+            b.set_mode(mode, -1)
+            b.set_launch(lanes, block, 0)
+            c, v, u, s = timed()
+            if c == T.OK and (best is None or v > best["value"]):
+                best = dict(value=v, us_per_step=u, launch=s)
+        per_mode[name] = best
+    b.close()
+    return dict(workload=f"{n_replicas} replicas of {n} particles (total {int(sizes.sum())}), {steps} steps",
+                value=value, unit="particle-steps/s", status=code, us_per_step=us, launch=shape,
+                best_resident=per_mode["resident"], best_tiled=per_mode["tiled"],
+                resident_over_tiled=per_mode["resident"]["value"] / per_mode["tiled"]["value"])
+
+
 def adjoint(T, mesh, n=1000, steps=200, ckpt=20):
     """Forward + adjoint: 1,000 particles, 200 steps, noise seed 2, checkpoints every 20 steps,
     loss (phi_T - 0.8)^2 + 0.1 mean |rdot - u|^2 with the seeded field u (seed 4); gradients with
@@ -489,6 +537,7 @@ def main():
         leg(out, "plain_resident_kernel", resident)
     if not args.no_batched:
         leg(out, "batched", batched, T, mesh, args.batched_steps)
+        leg(out, "batched_sweep", batched_sweep, T, mesh, args.batched_steps)
     if not args.no_adjoint:
         log("forward + adjoint: 1000 particles, 200 steps")
         leg(out, "adjoint", adjoint, T, mesh)
