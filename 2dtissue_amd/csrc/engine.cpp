@@ -515,7 +515,10 @@ void Engine::enqueue(const Shape& s, int n_steps, const float* nz, size_t stride
                               n_steps, t0, stream));
     }
   }
-  HIP_OK(launch_flags_end(R, d_rows + (size_t)(n_steps % 3) * R, d_halt, stream));
+  // the flags and first failing steps go right after the run's order history (run_once reads
+  // both back with one copy)
+  unsigned* tail = reinterpret_cast<unsigned*>(phi + 2 * (size_t)R * n_steps);
+  HIP_OK(launch_flags_end(R, d_rows + (size_t)(n_steps % 3) * R, d_halt, d_first_bad, tail, stream));
 }
 
 int32_t Engine::run(int n_steps, const float* nz, size_t stride, int row0) {
@@ -546,12 +549,13 @@ int32_t Engine::run_once(const Shape& s, int n_steps, const float* nz, size_t st
   if (n_steps < 0) throw ApiError(TISSUE_E_INVALID_ARG, "negative step count");
   if (n_steps == 0) return TISSUE_OK;
   const size_t need_phi = 2 * (size_t)R * n_steps;
-  if (phi_cap < need_phi) {
+  const size_t need_back = need_phi + (size_t)R;  // + the flag words (2R of 4 bytes) written by flags_end
+  if (phi_cap < need_back) {
     HIP_OK(hipStreamSynchronize(stream));
     drop_graphs();
     dfree(phi);
-    phi = dalloc<double>(need_phi);
-    phi_cap = need_phi;
+    phi = dalloc<double>(need_back);
+    phi_cap = need_back;
   }
   if (s.mode == TISSUE_MODE_STEP_KERNEL) {
     ensure_buckets();
@@ -597,7 +601,7 @@ int32_t Engine::run_once(const Shape& s, int n_steps, const float* nz, size_t st
   HIP_OK(hipGraphLaunch(it->second, stream));
   HIP_OK(hipEventRecord(ev1, stream));
   // read back into pinned memory (direct copies; pageable ones are staged by the runtime)
-  const size_t back = need_phi * sizeof(double) + 2 * (size_t)R * sizeof(int);
+  const size_t back = need_back * sizeof(double);
   if (h_back_cap < back) {
     HIP_OK(hipStreamSynchronize(stream));
     if (h_back) (void)hipHostFree(h_back);
@@ -609,9 +613,7 @@ int32_t Engine::run_once(const Shape& s, int n_steps, const float* nz, size_t st
   const double* ph = reinterpret_cast<const double*>(h_back);
   const unsigned* fl = reinterpret_cast<const unsigned*>(h_back + need_phi * sizeof(double));
   const int* fb = reinterpret_cast<const int*>(h_back + need_phi * sizeof(double) + (size_t)R * sizeof(int));
-  HIP_OK(hipMemcpyAsync((void*)ph, phi, need_phi * sizeof(double), hipMemcpyDeviceToHost, stream));
-  HIP_OK(hipMemcpyAsync((void*)fl, d_halt, R * sizeof(unsigned), hipMemcpyDeviceToHost, stream));
-  HIP_OK(hipMemcpyAsync((void*)fb, d_first_bad, R * sizeof(int), hipMemcpyDeviceToHost, stream));
+  HIP_OK(hipMemcpyAsync((void*)ph, phi, back, hipMemcpyDeviceToHost, stream));  // history, flags, first steps
   HIP_OK(hipStreamSynchronize(stream));
   float ms = 0.f;
   HIP_OK(hipEventElapsedTime(&ms, ev0, ev1));

@@ -320,9 +320,15 @@ __global__ void k_flags_begin(int R, const unsigned* halt, unsigned* rows, int* 
   first_bad[r] = INT_MAX;
 }
 
-__global__ void k_flags_end(int R, const unsigned* last_row, unsigned* halt) {
+__global__ void k_flags_end(int R, const unsigned* last_row, unsigned* halt, const int* first_bad, unsigned* tail) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r < R) halt[r] = last_row[r];
+  if (r >= R) return;
+  const unsigned f = last_row[r];
+  halt[r] = f;
+  if (tail) {  // the run's flags and first failing steps, beside its order history, for one read-back
+    tail[r] = f;
+    tail[R + r] = (unsigned)first_bad[r];
+  }
 }
 
 __global__ void k_prepare(int n, const float* x, const float* y, const float* th, float4* xycs, float* th_out) {
@@ -507,8 +513,9 @@ hipError_t launch_flags_begin(int R, const unsigned* halt, unsigned* rows, int* 
   return hipGetLastError();
 }
 
-hipError_t launch_flags_end(int R, const unsigned* last_row, unsigned* halt, hipStream_t st) {
-  hipLaunchKernelGGL(k_flags_end, dim3(cdiv(R, 64)), dim3(64), 0, st, R, last_row, halt);
+hipError_t launch_flags_end(int R, const unsigned* last_row, unsigned* halt, const int* first_bad, unsigned* tail,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_flags_end, dim3(cdiv(R, 64)), dim3(64), 0, st, R, last_row, halt, first_bad, tail);
   return hipGetLastError();
 }
 

@@ -181,7 +181,10 @@ hipError_t launch_phi_tiles(int R, int steps, const int* tile_begin, int n_logic
 inline int partials_per_tile(int block) { return block / 64; }
 // run start: rows[0] = halt, rows[1] = 0, first_bad = INT_MAX; run end: halt = last row
 hipError_t launch_flags_begin(int R, const unsigned* halt, unsigned* rows, int* first_bad, hipStream_t st);
-hipError_t launch_flags_end(int R, const unsigned* last_row, unsigned* halt, hipStream_t st);
+// halt = the run's last flag row; with tail, also tail[0 .. R) = the flags and tail[R .. 2R) =
+// first_bad (the run's read-back words)
+hipError_t launch_flags_end(int R, const unsigned* last_row, unsigned* halt, const int* first_bad, unsigned* tail,
+                            hipStream_t st);
 
 hipError_t launch_prepare(int n, const float* x, const float* y, const float* th, float4* xycs, float* th_out,
                           hipStream_t st);
