@@ -96,6 +96,8 @@ Engine::Engine(tissue_mesh* m, int R_, const int* n_, const tissue_params* p_) {
     d_rows = dalloc<unsigned>(3 * (size_t)R);
     d_first_bad = dalloc<int>(R);
     HIP_OK(hipMemsetAsync(d_halt, 0, R * sizeof(unsigned), stream));
+    // the flag rows as a run expects them at its start (each run's end leaves them so again)
+    HIP_OK(launch_flags_begin(R, d_halt, d_rows, d_first_bad, stream));
     HIP_OK(hipStreamSynchronize(stream));
     hist_ref.resize(R);
     hist_polar.resize(R);
@@ -456,7 +458,9 @@ Shape Engine::resolve_for(Shape s) const {
 }
 
 void Engine::enqueue(const Shape& s, int n_steps, const float* nz, size_t stride) {
-  HIP_OK(launch_flags_begin(R, d_halt, d_rows, d_first_bad, stream));
+  // flag rows rotate by step (t % 3); the first step reads the flags raised before the run from
+  // d_halt itself, and finds its output row cleared and first_bad reset by the previous run's end
+  // (or the construction)
   if (s.mode == TISSUE_MODE_RESIDENT) {
     ResidentIO io;
     const int cur = sphase;
@@ -473,7 +477,7 @@ void Engine::enqueue(const Shape& s, int n_steps, const float* nz, size_t stride
       io.noise = nz ? nz + (size_t)t0 * stride : nullptr;
       io.noise_row0 = d_row0;
       io.t_base = t0;
-      io.flags_in = d_rows + (size_t)(t0 % 3) * R;
+      io.flags_in = t0 == 0 ? d_halt : d_rows + (size_t)(t0 % 3) * R;
       io.flags_out = d_rows + (size_t)((t0 + k) % 3) * R;
       HIP_OK(launch_resident(s.lanes, s.block, md, d_reps, R, nmax, k, io, stream));
     }
@@ -492,7 +496,7 @@ void Engine::enqueue(const Shape& s, int n_steps, const float* nz, size_t stride
         io.noise_row0 = d_row0;
         io.noise_stride = stride;
         io.partial = partial + (size_t)q * n_logical * partials_per_tile(s.block);
-        io.halt_in = d_rows + (size_t)(t % 3) * R;
+        io.halt_in = t == 0 ? d_halt : d_rows + (size_t)(t % 3) * R;
         io.halt_out = d_rows + (size_t)((t + 1) % 3) * R;
         io.halt_clr = d_rows + (size_t)((t + 2) % 3) * R;
         io.first_bad = d_first_bad;
@@ -518,7 +522,7 @@ void Engine::enqueue(const Shape& s, int n_steps, const float* nz, size_t stride
   // the flags and first failing steps go right after the run's order history (run_once reads
   // both back with one copy)
   unsigned* tail = reinterpret_cast<unsigned*>(phi + 2 * (size_t)R * n_steps);
-  HIP_OK(launch_flags_end(R, d_rows + (size_t)(n_steps % 3) * R, d_halt, d_first_bad, tail, stream));
+  HIP_OK(launch_flags_end(R, d_rows + (size_t)(n_steps % 3) * R, d_halt, d_first_bad, tail, d_rows + R, stream));
 }
 
 int32_t Engine::run(int n_steps, const float* nz, size_t stride, int row0) {

@@ -320,15 +320,21 @@ __global__ void k_flags_begin(int R, const unsigned* halt, unsigned* rows, int* 
   first_bad[r] = INT_MAX;
 }
 
-__global__ void k_flags_end(int R, const unsigned* last_row, unsigned* halt, const int* first_bad, unsigned* tail) {
+__global__ void k_flags_end(int R, const unsigned* last_row, unsigned* halt, int* first_bad, unsigned* tail,
+                            unsigned* next_out_row) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= R) return;
   const unsigned f = last_row[r];
+  const int fb = first_bad[r];
   halt[r] = f;
   if (tail) {  // the run's flags and first failing steps, beside its order history, for one read-back
     tail[r] = f;
-    tail[R + r] = (unsigned)first_bad[r];
+    tail[R + r] = (unsigned)fb;
   }
+  // what k_flags_begin would set for the next run (which reads halt itself at its first step):
+  // the first step's output row cleared, no failure yet
+  next_out_row[r] = 0u;
+  first_bad[r] = INT_MAX;
 }
 
 __global__ void k_prepare(int n, const float* x, const float* y, const float* th, float4* xycs, float* th_out) {
@@ -513,9 +519,10 @@ hipError_t launch_flags_begin(int R, const unsigned* halt, unsigned* rows, int* 
   return hipGetLastError();
 }
 
-hipError_t launch_flags_end(int R, const unsigned* last_row, unsigned* halt, const int* first_bad, unsigned* tail,
-                            hipStream_t st) {
-  hipLaunchKernelGGL(k_flags_end, dim3(cdiv(R, 64)), dim3(64), 0, st, R, last_row, halt, first_bad, tail);
+hipError_t launch_flags_end(int R, const unsigned* last_row, unsigned* halt, int* first_bad, unsigned* tail,
+                            unsigned* next_out_row, hipStream_t st) {
+  hipLaunchKernelGGL(k_flags_end, dim3(cdiv(R, 64)), dim3(64), 0, st, R, last_row, halt, first_bad, tail,
+                     next_out_row);
   return hipGetLastError();
 }
 

@@ -182,9 +182,10 @@ inline int partials_per_tile(int block) { return block / 64; }
 // run start: rows[0] = halt, rows[1] = 0, first_bad = INT_MAX; run end: halt = last row
 hipError_t launch_flags_begin(int R, const unsigned* halt, unsigned* rows, int* first_bad, hipStream_t st);
 // halt = the run's last flag row; with tail, also tail[0 .. R) = the flags and tail[R .. 2R) =
-// first_bad (the run's read-back words)
-hipError_t launch_flags_end(int R, const unsigned* last_row, unsigned* halt, const int* first_bad, unsigned* tail,
-                            hipStream_t st);
+// first_bad (the run's read-back words); then, for the next run, next_out_row (the output row of
+// its first step) = 0 and first_bad = INT_MAX
+hipError_t launch_flags_end(int R, const unsigned* last_row, unsigned* halt, int* first_bad, unsigned* tail,
+                            unsigned* next_out_row, hipStream_t st);
 
 hipError_t launch_prepare(int n, const float* x, const float* y, const float* th, float4* xycs, float* th_out,
                           hipStream_t st);
