@@ -131,12 +131,12 @@ void Engine::release() {
   dfree(vx); dfree(vy); dfree(r3x); dfree(r3y); dfree(r3z); dfree(face); dfree(nbr);
   dfree(d_halt); dfree(d_rows); dfree(d_first_bad);
   for (int k = 0; k < 3; ++k) free_bucket_set(bs[k]);
-  dfree(d_tiles); dfree(d_tile_begin); dfree(partial); dfree(phi); dfree(noise); dfree(call_noise); dfree(d_row0);
+  dfree(d_tiles); dfree(d_tile_begin); dfree(partial); dfree(noise); dfree(call_noise); dfree(d_row0);
+  if (phi) (void)hipHostFree(phi);
+  phi = phi_dev = nullptr;
+  phi_cap = 0;
   if (h_row0) (void)hipHostFree(h_row0);
   h_row0 = nullptr;
-  if (h_back) (void)hipHostFree(h_back);
-  h_back = nullptr;
-  h_back_cap = 0;
   adjoint_release();
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
@@ -469,7 +469,7 @@ void Engine::enqueue(const Shape& s, int n_steps, const float* nz, size_t stride
     io.vx = vx; io.vy = vy; io.r3x = r3x; io.r3y = r3y; io.r3z = r3z;
     io.face = face; io.nbr = nbr;
     io.noise_stride = stride;
-    io.phi = phi;
+    io.phi = phi_dev;
     io.phi_stride = n_steps;
     io.first_bad = d_first_bad;
     for (int t0 = 0; t0 < n_steps; t0 += s.spl) {
@@ -515,13 +515,13 @@ void Engine::enqueue(const Shape& s, int n_steps, const float* nz, size_t stride
         sp = 1 - sp;
         bp = (bp + 1) % 3;
       }
-      HIP_OK(launch_phi_tiles(R, k, d_tile_begin, n_logical, partials_per_tile(s.block), partial, d_reps, phi,
+      HIP_OK(launch_phi_tiles(R, k, d_tile_begin, n_logical, partials_per_tile(s.block), partial, d_reps, phi_dev,
                               n_steps, t0, stream));
     }
   }
-  // the flags and first failing steps go right after the run's order history (run_once reads
-  // both back with one copy)
-  unsigned* tail = reinterpret_cast<unsigned*>(phi + 2 * (size_t)R * n_steps);
+  // the flags and first failing steps go right after the run's order history, in the pinned
+  // buffer that run_once reads
+  unsigned* tail = reinterpret_cast<unsigned*>(phi_dev + 2 * (size_t)R * n_steps);
   HIP_OK(launch_flags_end(R, d_rows + (size_t)(n_steps % 3) * R, d_halt, d_first_bad, tail, d_rows + R, stream));
 }
 
@@ -557,9 +557,12 @@ int32_t Engine::run_once(const Shape& s, int n_steps, const float* nz, size_t st
   if (phi_cap < need_back) {
     HIP_OK(hipStreamSynchronize(stream));
     drop_graphs();
-    dfree(phi);
-    phi = dalloc<double>(need_back);
+    if (phi) (void)hipHostFree(phi);
+    phi = phi_dev = nullptr;
+    phi_cap = 0;
+    HIP_OK(hipHostMalloc((void**)&phi, need_back * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
     phi_cap = need_back;
+    HIP_OK(hipHostGetDevicePointer((void**)&phi_dev, phi, 0));
   }
   if (s.mode == TISSUE_MODE_STEP_KERNEL) {
     ensure_buckets();
@@ -598,27 +601,22 @@ int32_t Engine::run_once(const Shape& s, int n_steps, const float* nz, size_t st
     it = graphs.emplace(key, ge).first;
   }
   // the previous run has completed (synchronised below), so the pinned words are free
-  h_row0[0] = nz ? row0 : 0;
-  h_row0[1] = (int)std::min<long>(steps_done - (rec ? rec->origin : 0), INT_MAX);
-  HIP_OK(hipMemcpyAsync(d_row0, h_row0, 2 * sizeof(int), hipMemcpyHostToDevice, stream));
+  const int w0 = nz ? row0 : 0;
+  const int w1 = rec ? (int)std::min<long>(steps_done - rec->origin, INT_MAX) : 0;
+  if (!row0_known || h_row0[0] != w0 || h_row0[1] != w1) {
+    h_row0[0] = w0;
+    h_row0[1] = w1;
+    HIP_OK(hipMemcpyAsync(d_row0, h_row0, 2 * sizeof(int), hipMemcpyHostToDevice, stream));
+    row0_known = true;
+  }
   HIP_OK(hipEventRecord(ev0, stream));
   HIP_OK(hipGraphLaunch(it->second, stream));
   HIP_OK(hipEventRecord(ev1, stream));
-  // read back into pinned memory (direct copies; pageable ones are staged by the runtime)
-  const size_t back = need_back * sizeof(double);
-  if (h_back_cap < back) {
-    HIP_OK(hipStreamSynchronize(stream));
-    if (h_back) (void)hipHostFree(h_back);
-    h_back = nullptr;
-    h_back_cap = 0;
-    HIP_OK(hipHostMalloc((void**)&h_back, back, hipHostMallocDefault));
-    h_back_cap = back;
-  }
-  const double* ph = reinterpret_cast<const double*>(h_back);
-  const unsigned* fl = reinterpret_cast<const unsigned*>(h_back + need_phi * sizeof(double));
-  const int* fb = reinterpret_cast<const int*>(h_back + need_phi * sizeof(double) + (size_t)R * sizeof(int));
-  HIP_OK(hipMemcpyAsync((void*)ph, phi, back, hipMemcpyDeviceToHost, stream));  // history, flags, first steps
   HIP_OK(hipStreamSynchronize(stream));
+  // the history, flags and first failing steps as the run's kernels wrote them
+  const double* ph = phi;
+  const unsigned* fl = reinterpret_cast<const unsigned*>(phi + need_phi);
+  const int* fb = reinterpret_cast<const int*>(phi + need_phi) + R;
   float ms = 0.f;
   HIP_OK(hipEventElapsedTime(&ms, ev0, ev1));
   last_ms = ms;

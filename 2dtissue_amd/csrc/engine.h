@@ -83,11 +83,11 @@ struct Engine {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // control words of the current run, read by its kernels: [0] first noise row, [1] steps done
-  // before the run (numbering of recorded steps); uploaded from pinned host memory
+  // since the output stream's origin (0 when not recording); uploaded from pinned host memory
+  // only when they differ from what the device holds (h_row0 mirrors it once row0_known)
   int* d_row0 = nullptr;
   int* h_row0 = nullptr;
-  unsigned char* h_back = nullptr;  // pinned read-back of a run: order history, flag words, first steps
-  size_t h_back_cap = 0;
+  bool row0_known = false;
   // per-step output stream (engine_record.cpp); null when not recording
   std::unique_ptr<struct Recorder> rec;
   void record_start(const char* dir, int every);
@@ -124,7 +124,12 @@ struct Engine {
   OrderPartial* partial = nullptr;
   size_t partial_cap = 0;
 
+  // a run's order history (2 doubles per replica and step), then its flag words and first
+  // failing steps (R each): coherent pinned host memory that the run's kernels write directly,
+  // read by the host once the run has synchronised (no read-back copy); phi_dev is its device
+  // address
   double* phi = nullptr;
+  double* phi_dev = nullptr;
   size_t phi_cap = 0;
   float* noise = nullptr;  // staged rows
   int noise_rows = 0;      // rows of the last staging (the valid ones)
