@@ -199,14 +199,25 @@ __global__ void __launch_bounds__(kStepMaxBlock) k_bucket_finalize(const Replica
   if (crowded == 0ull) return;  // uniform over the wave
   const int2* ovf = bs.ovf + off;
   int* big = bs.big + off;
+  // the ranges of all the wave's crowded buckets reserved with one atomic, in lane order (a
+  // returning device-scope atomic is a memory-side round trip; one per bucket serialised them)
+  int pre = c > kSlots ? c : 0;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int o = __shfl_up(pre, d, 64);
+    if (lane >= d) pre += o;
+  }
+  const int total = __shfl(pre, 63, 64);
+  int base0 = 0;
+  if (lane == 0) base0 = atomicAdd(&bs.big_n[r], total);
+  base0 = __shfl(base0, 0, 64);
+  const int my_start = base0 + pre - (c > kSlots ? c : 0);
   while (crowded) {
     const int src = __ffsll((long long)crowded) - 1;
     crowded &= crowded - 1ull;
     const int w = __shfl(v, src, 64);
     const int cw = __shfl(c, src, 64);
-    // reserve the bucket's range first: the reservation is in flight during the scan below
-    int reserved = 0;
-    if (lane == 0) reserved = atomicAdd(&bs.big_n[r], cw);
+    const int reserved = __shfl(my_start, src, 64);
     int* sl = slot + (size_t)w * kSlots;
     const bool in_lds = cw <= kSortCap - 4;  // room for the padding below
     if (in_lds) {
@@ -232,7 +243,7 @@ __global__ void __launch_bounds__(kStepMaxBlock) k_bucket_finalize(const Replica
       }
       if (lane < 4) buf[cw + lane] = 0x7fffffff;  // pad to a multiple of four (never below an id)
       wave_sync();
-      int* dst = big + __shfl(reserved, 0, 64);
+      int* dst = big + reserved;
       // rank of each id among the bucket's (distinct) ids = its position in ascending order; the
       // ids are read four at a time, several reads in flight
       const int cw4 = (cw + 3) & ~3;
@@ -248,7 +259,7 @@ __global__ void __launch_bounds__(kStepMaxBlock) k_bucket_finalize(const Replica
       }
     } else {
       // rare: gathered and sorted in place in global memory
-      int* dst = big + __shfl(reserved, 0, 64);
+      int* dst = big + reserved;
       if (lane < kSlots) dst[lane] = sl[lane];
       int fill = kSlots;
       for (int q0 = 0; q0 < novf; q0 += 64) {
