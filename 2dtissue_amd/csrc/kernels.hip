@@ -180,10 +180,12 @@ __global__ void __launch_bounds__(kStepMaxBlock) k_bucket_finalize(const Replica
   const int v_listed = k < cap ? bs.multi[off + k] : -1;
   int* slot = bs.slot + (size_t)r * nv * kSlots;
   int v = -1, c = 0;
+  int4 s0 = make_int4(0, 0, 0, 0), s1 = s0;
   if (k < n_multi) {
     v = v_listed;
     int* sl = slot + (size_t)v * kSlots;
-    const int4 s0 = reinterpret_cast<const int4*>(sl)[0], s1 = reinterpret_cast<const int4*>(sl)[1];
+    s0 = reinterpret_cast<const int4*>(sl)[0];
+    s1 = reinterpret_cast<const int4*>(sl)[1];
     c = bs.count[(size_t)r * nv + v];
     if (c <= kSlots) {
       int a[kSlots] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
@@ -222,7 +224,14 @@ __global__ void __launch_bounds__(kStepMaxBlock) k_bucket_finalize(const Replica
     const bool in_lds = cw <= kSortCap - 4;  // room for the padding below
     if (in_lds) {
       int* buf = lds[wave];
-      if (lane < kSlots) buf[lane] = sl[lane];
+      // the bucket's slots as its lane loaded them above
+      const int f0 = __builtin_amdgcn_readlane(s0.x, src), f1 = __builtin_amdgcn_readlane(s0.y, src);
+      const int f2 = __builtin_amdgcn_readlane(s0.z, src), f3 = __builtin_amdgcn_readlane(s0.w, src);
+      const int f4 = __builtin_amdgcn_readlane(s1.x, src), f5 = __builtin_amdgcn_readlane(s1.y, src);
+      const int f6 = __builtin_amdgcn_readlane(s1.z, src), f7 = __builtin_amdgcn_readlane(s1.w, src);
+      if (lane < kSlots)
+        buf[lane] = lane < 4 ? (lane < 2 ? (lane == 0 ? f0 : f1) : (lane == 2 ? f2 : f3))
+                             : (lane < 6 ? (lane == 4 ? f4 : f5) : (lane == 6 ? f6 : f7));
       int fill = kSlots;
       // four chunks of the overflow list loaded together (one load latency per 256 entries), then
       // compacted in order
