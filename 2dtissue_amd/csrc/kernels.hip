@@ -214,6 +214,13 @@ __global__ void __launch_bounds__(kStepMaxBlock) k_bucket_finalize(const Replica
   if (lane == 0) base0 = atomicAdd(&bs.big_n[r], total);
   base0 = __shfl(base0, 0, 64);
   const int my_start = base0 + pre - (c > kSlots ? c : 0);
+  // the overflow list's first 256 entries, loaded once for all the wave's crowded buckets
+  int2 e0[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int q = 64 * u + lane;
+    e0[u] = q < novf ? ovf[q] : make_int2(-1, 0);
+  }
   while (crowded) {
     const int src = __ffsll((long long)crowded) - 1;
     crowded &= crowded - 1ull;
@@ -233,14 +240,14 @@ __global__ void __launch_bounds__(kStepMaxBlock) k_bucket_finalize(const Replica
         buf[lane] = lane < 4 ? (lane < 2 ? (lane == 0 ? f0 : f1) : (lane == 2 ? f2 : f3))
                              : (lane < 6 ? (lane == 4 ? f4 : f5) : (lane == 6 ? f6 : f7));
       int fill = kSlots;
-      // four chunks of the overflow list loaded together (one load latency per 256 entries), then
-      // compacted in order
+      // the overflow entries in chunks of 256 (four loads per lane in flight together; the first
+      // chunk loaded before the loop), compacted in order
       for (int q0 = 0; q0 < novf; q0 += 256) {
         int2 e[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const int q = q0 + 64 * u + lane;
-          e[u] = q < novf ? ovf[q] : make_int2(-1, 0);
+          e[u] = q0 == 0 ? e0[u] : (q < novf ? ovf[q] : make_int2(-1, 0));
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
